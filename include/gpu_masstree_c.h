@@ -1,0 +1,133 @@
+/* C ABI of the MI355X Masstree (libgpu_masstree.so).
+ *
+ * Plain pointers and sizes only.  Key/value/output pointers are device pointers (or
+ * pinned host pointers, as the reference's tests use).  `stream` is a hipStream_t
+ * (NULL = the default stream).  Batch calls are asynchronous on `stream`; every call
+ * returns 0 on success or a GMT_E* code for an invalid argument or a HIP failure
+ * (message: gmt_last_error_message()).  Errors found inside a kernel (pool exhausted,
+ * invalid key length, ...) are reported by gmt_device_error().
+ *
+ * The reference exposes this path as the C++ template API
+ * GpuMasstree::gpu_masstree<Allocator, Reclaimer, tile_size> (include/gpu_masstree.hpp
+ * in hyoungjook/IndexinGPU); each entry point below names the member it stands for.
+ * The same API is available here as C++ in <indexingpu_amd/gpu_masstree.hpp>.
+ */
+#ifndef GPU_MASSTREE_C_H
+#define GPU_MASSTREE_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gmt_index gmt_index;
+
+enum gmt_status {
+  GMT_OK = 0,
+  GMT_EINVAL = 1,   /* bad argument (null handle, unsupported tile size, ...) */
+  GMT_EHIP = 2,     /* a HIP runtime call failed */
+};
+
+/* Request types of gmt_mixed_batch: kernels::request_type (include/macros.hpp:51-58). */
+enum gmt_request_type {
+  GMT_REQUEST_INSERT = 0,
+  GMT_REQUEST_ERASE = 1,
+  GMT_REQUEST_FIND = 2,
+  GMT_REQUEST_UPDATE = 3,
+};
+
+/* Device-side error codes (gmt_device_error). */
+enum gmt_device_error_code {
+  GMT_DEV_OK = 0,
+  GMT_DEV_OUT_OF_MEMORY = 1,
+  GMT_DEV_STEP_LIMIT = 2,
+  GMT_DEV_BAD_INDEX = 3,
+  GMT_DEV_BAD_LENGTH = 4,
+  GMT_DEV_RETIRE_OVERFLOW = 5,
+  GMT_DEV_SCAN_SCRATCH = 6,
+};
+
+/* Creates an index with its own slab allocator and reclaimer.
+ * Replaces: simple_slab_allocator<128>(pool_ratio) (include/simple_slab_alloc.hpp:56-69),
+ *           simple_debra_reclaimer<>() (include/simple_debra_reclaim.hpp:56-67),
+ *           gpu_masstree(allocator, reclaimer) (include/gpu_masstree.hpp:67-72).
+ * tile_size: 16, 32 or 64.  pool_bytes > 0 sizes the slab pool explicitly; otherwise
+ * pool_ratio x total device memory is used (the reference's rule).  retire_capacity
+ * = 0 picks the default retire-list size. */
+int gmt_create(uint32_t tile_size, uint64_t pool_bytes, float pool_ratio,
+               uint32_t retire_capacity, gmt_index** out);
+int gmt_destroy(gmt_index* index);
+
+/* gpu_masstree::find<concurrent> (include/gpu_masstree.hpp:86-99). value_lengths[i] = 0
+ * means absent; values get min(length, max_value_length) slices. */
+int gmt_find(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
+             const uint32_t* key_lengths, uint32_t* values, uint32_t max_value_length,
+             uint32_t* value_lengths, uint32_t num_keys, int concurrent, void* stream);
+
+/* gpu_masstree::insert<update_if_exists> (include/gpu_masstree.hpp:101-115).
+ * results (nullable, one byte per request) receives per-op success: an extension. */
+int gmt_insert(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
+               const uint32_t* key_lengths, const uint32_t* values, uint32_t max_value_length,
+               const uint32_t* value_lengths, uint32_t num_keys, int update_if_exists,
+               uint8_t* results, void* stream);
+
+/* gpu_masstree::update (include/gpu_masstree.hpp:117-130). */
+int gmt_update(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
+               const uint32_t* key_lengths, const uint32_t* values, uint32_t max_value_length,
+               const uint32_t* value_lengths, uint32_t num_keys, uint8_t* results, void* stream);
+
+/* gpu_masstree::erase (include/gpu_masstree.hpp:132-145). */
+int gmt_erase(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
+              const uint32_t* key_lengths, uint32_t num_keys, uint8_t* results, void* stream);
+
+/* gpu_masstree::scan<use_upper_key> (include/gpu_masstree.hpp:147-171); use_upper_key is
+ * (upper_keys != NULL).  per_query_limits (nullable) caps query q at
+ * min(per_query_limits[q], max_count_per_query): an extension. */
+int gmt_scan(gmt_index* index, const uint32_t* lower_keys, const uint32_t* lower_key_lengths,
+             uint32_t max_key_length, uint32_t max_count_per_query, uint32_t num_queries,
+             const uint32_t* upper_keys, const uint32_t* upper_key_lengths, uint32_t* counts,
+             uint32_t* values, uint32_t max_value_length, uint32_t* value_lengths,
+             uint32_t* out_keys, uint32_t* out_key_lengths, const uint32_t* per_query_limits,
+             void* stream);
+
+/* gpu_masstree::mixed_batch (include/gpu_masstree.hpp:173-191). types[i] is a
+ * gmt_request_type; finds write values/value_lengths in place, writes set results[i]. */
+int gmt_mixed_batch(gmt_index* index, const uint8_t* types, const uint32_t* keys,
+                    uint32_t max_key_length, const uint32_t* key_lengths, uint32_t* values,
+                    uint32_t max_value_length, uint32_t* value_lengths, uint8_t* results,
+                    uint32_t num_requests, void* stream);
+
+/* Waits for the device and returns (then clears, if clear != 0) the first device-side
+ * error code since the last clear (gmt_device_error_code). */
+uint32_t gmt_device_error(gmt_index* index, int clear);
+
+/* Host walk of the whole tree (debug; the reference's validate(),
+ * include/gpu_masstree.hpp:1421-1472).  Returns 1 if every invariant holds, 0 if not
+ * (first violation in msg), negative on failure.  Any output pointer may be NULL. */
+int gmt_validate(gmt_index* index, uint64_t* entries, uint64_t* nodes, uint64_t* record_slabs,
+                 char* msg, uint32_t msg_len);
+
+/* Dumps every entry in key order into host buffers (debug and parity checks).  Call
+ * with NULL buffers to get the totals: returns the entry count (or -1), sets
+ * *key_slices / *value_slices.  key_offsets / value_offsets hold count + 1 entries. */
+int64_t gmt_dump(gmt_index* index, uint32_t* key_offsets, uint32_t* key_data,
+                 uint32_t* value_offsets, uint32_t* value_data, uint64_t* key_slices,
+                 uint64_t* value_slices);
+
+/* Slabs currently allocated (the reference's num_nodes_used(),
+ * include/gpu_masstree.hpp:1415-1418) and the pool size in slabs. */
+uint64_t gmt_slabs_allocated(gmt_index* index);
+uint64_t gmt_pool_slabs(gmt_index* index);
+
+/* Tile width the index was created with. */
+uint32_t gmt_tile_size(gmt_index* index);
+
+/* Message of the last failing call on this thread. */
+const char* gmt_last_error_message(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPU_MASSTREE_C_H */

@@ -1,0 +1,460 @@
+// GpuMasstree::gpu_masstree for MI355X (gfx950): the reference's drop-in API
+// (include/gpu_masstree.hpp:43-191 host calls, :194-971 device calls) over this
+// project's node layout, state machine and kernels.
+//
+// Host calls are asynchronous on `stream` and take device (or pinned host) buffers,
+// exactly as the reference's.  Extensions, all defaulted so existing call sites
+// compile unchanged:
+//   * insert/update/erase accept `bool* results` after `stream` (per-op success);
+//   * scan accepts a per-query limit array after `stream`;
+//   * tile_size may be 16, 32 or 64;
+//   * last_error(), drain(), validate_tree(), dump_entries() for tests and tools.
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "hip_util.hpp"
+#include "host_walk.hpp"
+#include "kernels.hpp"
+#include "layout.hpp"
+#include "reclaim.hpp"
+#include "slab_alloc.hpp"
+#include "tile_policy.hpp"
+#include "tree_ops.hpp"
+
+namespace kernels {
+// include/macros.hpp:51-58 (values are part of the drop-in ABI)
+enum request_type : uint8_t {
+  request_type_insert = 0,
+  request_type_erase = 1,
+  request_type_find = 2,
+  request_type_update = 3
+};
+}  // namespace kernels
+
+namespace indexingpu_amd {
+
+// Device-side contexts (the reference's device_allocator_context<Alloc> /
+// device_reclaimer_context<Reclaimer>, include/device_context.hpp:18-21).  They carry a
+// tile's cached bitmap word and retire reservation between cooperative calls; a tile
+// returns what is left when its context is destroyed.
+template <class Alloc>
+struct device_allocator_context {
+  slab_pool_view view;
+  tile_slab_cache cache;
+  template <class Tile>
+  __device__ device_allocator_context(const slab_pool_view& v, const Tile&) : view(v) {}
+  __device__ ~device_allocator_context() {
+    if (cache.bits != 0 && (__lane_id() & 15u) == 0)
+      dev::atomic_and_relaxed(&view.bitmap[cache.word], ~cache.bits);
+  }
+};
+
+template <class Reclaimer>
+struct device_reclaimer_context {
+  retire_view view;
+  tile_retire_cache cache;
+  template <class Tile>
+  __device__ device_reclaimer_context(const retire_view& v, const Tile&) : view(v) {}
+  __device__ ~device_reclaimer_context() {
+    if (cache.base != kInvalid && (__lane_id() & 15u) == 0)
+      for (u32 j = cache.used; j < kRetireChunk; j++) view.list[cache.base + j] = kInvalid;
+  }
+};
+
+inline u32 device_cu_count() {
+  int dev = 0, cus = 0;
+  IGPU_HIP_TRY(hipGetDevice(&dev));
+  IGPU_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return (u32)cus;
+}
+
+// Resident-block capacity of a kernel at kBlockThreads, computed once per kernel.
+template <class K>
+inline u32 resident_blocks(K kernel) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, u32> cache;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find(reinterpret_cast<const void*>(kernel));
+  if (it != cache.end()) return it->second;
+  int per_cu = 0;
+  IGPU_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlockThreads, 0));
+  if (per_cu < 1) per_cu = 1;
+  u32 v = (u32)per_cu * device_cu_count();
+  cache[reinterpret_cast<const void*>(kernel)] = v;
+  return v;
+}
+
+}  // namespace indexingpu_amd
+
+namespace GpuMasstree {
+
+using indexingpu_amd::device_allocator_context;
+using indexingpu_amd::device_reclaimer_context;
+
+template <typename Allocator, typename Reclaimer, uint32_t tile_size = 16>
+struct gpu_masstree {
+  using size_type = uint32_t;
+  using key_slice_type = uint32_t;
+  using value_slice_type = uint32_t;
+  static constexpr uint32_t tile_size_ = tile_size;
+  static_assert(tile_size == 16 || tile_size == 32 || tile_size == 64,
+                "tile_size must be 16, 32 or 64 lanes");
+  static constexpr auto branching_factor = 16;
+  static constexpr size_type invalid_pointer = 0xFFFFFFFFu;
+
+  using host_allocator_type = Allocator;
+  using device_allocator_instance_type = typename Allocator::device_instance_type;
+  using device_allocator_context_type = device_allocator_context<Allocator>;
+  using host_reclaimer_type = Reclaimer;
+  using device_reclaimer_instance_type = typename Reclaimer::device_instance_type;
+  using device_reclaimer_context_type = device_reclaimer_context<Reclaimer>;
+
+  gpu_masstree() = delete;
+  gpu_masstree(const host_allocator_type& host_allocator, const host_reclaimer_type& host_reclaimer)
+      : allocator_(host_allocator.get_device_instance()),
+        reclaimer_(host_reclaimer.get_device_instance()) {
+    allocate();
+  }
+  // shallow copy (kernels receive the tree by value, as in the reference)
+  gpu_masstree(const gpu_masstree& other)
+      : root_index_(other.root_index_),
+        allocator_(other.allocator_),
+        reclaimer_(other.reclaimer_),
+        err_(other.err_),
+        owner_(false) {}
+  gpu_masstree& operator=(const gpu_masstree&) = delete;
+  ~gpu_masstree() {
+    if (owner_) {
+      if (err_) (void)hipFree(err_);
+      if (scan_stack_) (void)hipFree(scan_stack_);
+    }
+  }
+
+  // ------------------------------------------------------------------ host API
+  // if key_lengths == nullptr, every key has max_key_length slices
+  template <bool concurrent = false, bool use_shmem_key = false>
+  void find(const key_slice_type* keys, const size_type max_key_length,
+            const size_type* key_lengths, value_slice_type* values,
+            const size_type max_value_length, size_type* value_lengths,
+            const size_type num_keys, hipStream_t stream = 0) {
+    indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
+    a.kind = indexingpu_amd::op_find;
+    a.max_value_length = max_value_length;
+    a.out_values = values;
+    a.out_value_lengths = value_lengths;
+    launch_batch<concurrent>(a, stream, false);
+  }
+
+  template <bool update_if_exists = false, bool enable_suffix = true, bool use_shmem_key = false>
+  void insert(const key_slice_type* keys, const size_type max_key_length,
+              const size_type* key_lengths, const value_slice_type* values,
+              const size_type max_value_length, const size_type* value_lengths,
+              const size_type num_keys, hipStream_t stream = 0, bool* results = nullptr) {
+    static_assert(enable_suffix, "layers without suffix records are not provided");
+    indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
+    a.kind = update_if_exists ? indexingpu_amd::op_upsert : indexingpu_amd::op_insert;
+    a.values = values;
+    a.max_value_length = max_value_length;
+    a.value_lengths = value_lengths;
+    a.results = results;
+    launch_batch<true>(a, stream, true);
+  }
+
+  template <bool enable_suffix = true, bool use_shmem_key = false>
+  void update(const key_slice_type* keys, const size_type max_key_length,
+              const size_type* key_lengths, const value_slice_type* values,
+              const size_type max_value_length, const size_type* value_lengths,
+              const size_type num_keys, hipStream_t stream = 0, bool* results = nullptr) {
+    indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
+    a.kind = indexingpu_amd::op_update;
+    a.values = values;
+    a.max_value_length = max_value_length;
+    a.value_lengths = value_lengths;
+    a.results = results;
+    launch_batch<true>(a, stream, true);
+  }
+
+  // Erase never merges nodes here (see tree_ops.hpp), so the merge switches of the
+  // reference's signature are accepted and have no effect.
+  template <bool do_remove_empty_root = true, bool pessimistic_merge = true, bool do_merge = true,
+            bool concurrent = true, bool use_shmem_key = false>
+  void erase(const key_slice_type* keys, const size_type max_key_length,
+             const size_type* key_lengths, const size_type num_keys, hipStream_t stream = 0,
+             bool* results = nullptr) {
+    indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
+    a.kind = indexingpu_amd::op_erase;
+    a.results = results;
+    launch_batch<true>(a, stream, true);
+  }
+
+  template <bool use_upper_key = true, bool concurrent = false, bool use_shmem_key = false>
+  void scan(const key_slice_type* lower_keys, const size_type* lower_key_lengths,
+            const size_type max_key_length, const size_type max_count_per_query,
+            const size_type num_queries, const key_slice_type* upper_keys = nullptr,
+            const size_type* upper_key_lengths = nullptr, size_type* counts = nullptr,
+            value_slice_type* values = nullptr, const size_type max_value_length = 1,
+            size_type* value_lengths = nullptr, key_slice_type* out_keys = nullptr,
+            size_type* out_key_lengths = nullptr, hipStream_t stream = 0,
+            const size_type* per_query_limits = nullptr) {
+    if (num_queries == 0) return;
+    indexingpu_amd::scan_args a{};
+    a.lower_keys = lower_keys;
+    a.lower_key_lengths = lower_key_lengths;
+    a.max_key_length = max_key_length;
+    a.max_count = max_count_per_query;
+    a.num_queries = num_queries;
+    a.upper_keys = use_upper_key ? upper_keys : nullptr;
+    a.upper_key_lengths = use_upper_key ? upper_key_lengths : nullptr;
+    a.counts = counts;
+    a.values = values;
+    a.max_value_length = max_value_length;
+    a.value_lengths = value_lengths;
+    a.out_keys = out_keys;
+    a.out_key_lengths = out_key_lengths;
+    a.limits = per_query_limits;
+    auto kernel = indexingpu_amd::scan_kernel<tile_size, concurrent>;
+    const uint32_t tiles_per_block = indexingpu_amd::kBlockThreads / tile_size;
+    uint64_t want = (num_queries + tiles_per_block - 1) / tiles_per_block;
+    uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(kernel));
+    a.stack_layers = std::max<uint32_t>(max_key_length, 64u) + 1;
+    ensure_scan_stack((uint64_t)blocks * tiles_per_block * a.stack_layers * 3u);
+    a.stack = scan_stack_;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
+                       view(), a);
+  }
+
+  template <bool enable_suffix = true, bool erase_do_remove_empty_root = true,
+            bool erase_pessimistic_merge = true, bool erase_do_merge = true,
+            bool use_shmem_key = false>
+  void mixed_batch(const kernels::request_type* request_types, const key_slice_type* keys,
+                   const size_type max_key_length, const size_type* key_lengths,
+                   value_slice_type* values, const size_type max_value_length,
+                   size_type* value_lengths, bool* results, const size_type num_requests,
+                   hipStream_t stream = 0) {
+    indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_requests);
+    a.types = reinterpret_cast<const uint8_t*>(request_types);
+    a.kind = indexingpu_amd::op_find;
+    a.values = values;
+    a.max_value_length = max_value_length;
+    a.value_lengths = value_lengths;
+    a.out_values = values;
+    a.out_value_lengths = value_lengths;
+    a.results = results;
+    launch_batch<true>(a, stream, true);
+  }
+
+  // ---------------------------------------------------------------- device API
+  // Every lane of the tile calls these with the same arguments.  Keys and values are
+  // plain slice pointers.  Each call runs the same step machine as the batch kernels.
+  template <bool concurrent, typename tile_type, typename K, typename V>
+  __device__ size_type cooperative_find(K& key, size_type key_length, V& value,
+                                        size_type max_value_length, const tile_type& tile,
+                                        device_allocator_context_type& allocator) {
+    indexingpu_amd::wave_tile<tile_size> c(allocator.view, indexingpu_amd::retire_view{}, err_);
+    indexingpu_amd::request rq{indexingpu_amd::op_find, key_length, as_ptr(key), nullptr, 0,
+                               as_mut_ptr(value), max_value_length};
+    return run_op<concurrent>(c, rq);
+  }
+
+  template <bool update_if_exists = false, bool enable_suffix = true, typename tile_type,
+            typename K, typename V>
+  __device__ bool cooperative_insert(K& key, const size_type key_length, V& value,
+                                     const size_type value_length, const tile_type& tile,
+                                     device_allocator_context_type& allocator,
+                                     device_reclaimer_context_type& reclaimer) {
+    indexingpu_amd::wave_tile<tile_size> c(allocator.view, reclaimer.view, err_);
+    c.ac = allocator.cache;
+    c.rc = reclaimer.cache;
+    indexingpu_amd::request rq{update_if_exists ? indexingpu_amd::op_upsert : indexingpu_amd::op_insert,
+                               key_length, as_ptr(key), as_ptr(value), value_length, nullptr, 0};
+    const bool r = run_op<true>(c, rq) != 0;
+    allocator.cache = c.ac;
+    reclaimer.cache = c.rc;
+    return r;
+  }
+
+  template <bool enable_suffix = true, typename tile_type, typename K, typename V>
+  __device__ bool cooperative_update(K& key, const size_type key_length, V& value,
+                                     const size_type value_length, const tile_type& tile,
+                                     device_allocator_context_type& allocator,
+                                     device_reclaimer_context_type& reclaimer) {
+    indexingpu_amd::wave_tile<tile_size> c(allocator.view, reclaimer.view, err_);
+    c.ac = allocator.cache;
+    c.rc = reclaimer.cache;
+    indexingpu_amd::request rq{indexingpu_amd::op_update, key_length, as_ptr(key), as_ptr(value),
+                               value_length, nullptr, 0};
+    const bool r = run_op<true>(c, rq) != 0;
+    allocator.cache = c.ac;
+    reclaimer.cache = c.rc;
+    return r;
+  }
+
+  template <bool concurrent, bool do_merge, bool pessimistic_merge, bool do_remove_empty_root,
+            typename tile_type, typename K>
+  __device__ bool cooperative_erase(K& key, const size_type key_length, const tile_type& tile,
+                                    device_allocator_context_type& allocator,
+                                    device_reclaimer_context_type& reclaimer) {
+    indexingpu_amd::wave_tile<tile_size> c(allocator.view, reclaimer.view, err_);
+    c.ac = allocator.cache;
+    c.rc = reclaimer.cache;
+    indexingpu_amd::request rq{indexingpu_amd::op_erase, key_length, as_ptr(key), nullptr, 0,
+                               nullptr, 0};
+    const bool r = run_op<true>(c, rq) != 0;
+    allocator.cache = c.ac;
+    reclaimer.cache = c.rc;
+    return r;
+  }
+
+  // -------------------------------------------------------------- host extras
+  // Waits for the device, returns the first error code since the last call (0 = none)
+  // and clears it.
+  uint32_t last_error() {
+    IGPU_HIP_TRY(hipDeviceSynchronize());
+    uint32_t h = 0;
+    IGPU_HIP_TRY(hipMemcpy(&h, err_, 4, hipMemcpyDeviceToHost));
+    IGPU_HIP_TRY(hipMemset(err_, 0, 4));
+    return h;
+  }
+
+  // Frees the records retired so far (batch calls do this themselves).
+  void drain(hipStream_t stream = 0) {
+    if (!reclaimer_.enabled) return;
+    hipLaunchKernelGGL(indexingpu_amd::drain_kernel, dim3(256), dim3(256), 0, stream,
+                       allocator_, reclaimer_);
+    hipLaunchKernelGGL(indexingpu_amd::reset_kernel, dim3(1), dim3(64), 0, stream, reclaimer_);
+  }
+
+  // Walks the whole tree on the host (copy of the used part of the pool).
+  indexingpu_amd::walk_stats walk(const indexingpu_amd::entry_emit_fn& emit) {
+    IGPU_HIP_TRY(hipDeviceSynchronize());
+    std::vector<uint32_t> bm(allocator_.num_words);
+    IGPU_HIP_TRY(hipMemcpy(bm.data(), allocator_.bitmap, 4ull * bm.size(), hipMemcpyDeviceToHost));
+    uint64_t top = 0;
+    for (uint64_t w = 0; w < bm.size(); w++)
+      if (bm[w]) top = w + 1;
+    std::vector<uint64_t> img(top * 32 * indexingpu_amd::kSlots);
+    if (top)
+      IGPU_HIP_TRY(hipMemcpy(img.data(), allocator_.pool, img.size() * 8, hipMemcpyDeviceToHost));
+    const uint64_t nslabs = top * 32;
+    indexingpu_amd::host_walker w(
+        [&](uint32_t idx) -> const uint64_t* {
+          return idx < nslabs ? img.data() + (uint64_t)idx * indexingpu_amd::kSlots : nullptr;
+        },
+        emit);
+    return w.run(root_index_);
+  }
+  bool validate_tree(std::string* msg = nullptr) {
+    auto s = walk(nullptr);
+    if (msg) *msg = s.error;
+    return s.error.empty();
+  }
+
+  // reference debug API (gpu_masstree.hpp:1411-1472)
+  void print() {
+    walk([](const std::vector<uint32_t>& k, const std::vector<uint32_t>& v) {
+      std::printf("key[");
+      for (size_t i = 0; i < k.size(); i++) std::printf(i ? " %u" : "%u", k[i]);
+      std::printf("] -> %zu value slices\n", v.size());
+    });
+  }
+  void validate() {
+    auto s = walk(nullptr);
+    if (!s.error.empty()) std::printf("validate: %s\n", s.error.c_str());
+    uint64_t total = (s.nodes + s.record_slabs) * indexingpu_amd::kSlabBytes;
+    std::printf("%lu entries, %lu nodes (+%lu suffix nodes) found\n", (unsigned long)s.entries,
+                (unsigned long)s.nodes, (unsigned long)s.record_slabs);
+    std::printf("Total Space Consumption: %lu B (%f B/entry)\n", (unsigned long)total,
+                s.entries ? (double)total / (double)s.entries : 0.0);
+  }
+  std::size_t num_nodes_used() { return allocator_.num_allocated_slabs(); }
+
+  size_type root_index_ = 0;
+  device_allocator_instance_type allocator_;
+  device_reclaimer_instance_type reclaimer_;
+  uint32_t* err_ = nullptr;
+
+ private:
+  template <class K>
+  __host__ __device__ static const uint32_t* as_ptr(K& k) {
+    return static_cast<const uint32_t*>(k);
+  }
+  template <class V>
+  __host__ __device__ static uint32_t* as_mut_ptr(V& v) {
+    return static_cast<uint32_t*>(v);
+  }
+
+  template <bool Validate>
+  __device__ size_type run_op(indexingpu_amd::wave_tile<tile_size>& c,
+                              const indexingpu_amd::request& rq) {
+    if (rq.key_length == 0) {
+      c.set_error(indexingpu_amd::kErrBadLength);
+      return 0;
+    }
+    c.load_key_cache(rq);
+    indexingpu_amd::op_state<indexingpu_amd::gpu_node> st;
+    indexingpu_amd::op_begin(c, st, root_index_);
+    while (st.phase != indexingpu_amd::ph_done) {
+      if (!indexingpu_amd::op_step<Validate>(c, rq, st)) __builtin_amdgcn_s_sleep(1);
+    }
+    return st.result;
+  }
+
+  indexingpu_amd::tree_view view() const {
+    return indexingpu_amd::tree_view{allocator_, reclaimer_, err_, root_index_};
+  }
+
+  static indexingpu_amd::batch_args base_args(const uint32_t* keys, uint32_t maxK,
+                                              const uint32_t* klens, uint64_t n) {
+    indexingpu_amd::batch_args a{};
+    a.keys = keys;
+    a.max_key_length = maxK;
+    a.key_lengths = klens;
+    a.n = n;
+    return a;
+  }
+
+  template <bool Validate>
+  void launch_batch(const indexingpu_amd::batch_args& a, hipStream_t stream, bool mutating) {
+    if (a.n == 0) return;
+    auto kernel = indexingpu_amd::batch_kernel<tile_size, Validate>;
+    const uint32_t tiles_per_block = indexingpu_amd::kBlockThreads / tile_size;
+    uint64_t want = (a.n + tiles_per_block - 1) / tiles_per_block;
+    uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(kernel));
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
+                       view(), a);
+    if (mutating) drain(stream);
+  }
+
+  void ensure_scan_stack(uint64_t words) {
+    if (words <= scan_stack_words_) return;
+    IGPU_HIP_TRY(hipDeviceSynchronize());
+    if (scan_stack_) IGPU_HIP_TRY(hipFree(scan_stack_));
+    IGPU_HIP_TRY(hipMalloc(&scan_stack_, words * 4));
+    scan_stack_words_ = words;
+  }
+
+  void allocate() {
+    IGPU_HIP_TRY(hipMalloc(&err_, 256));
+    IGPU_HIP_TRY(hipMemset(err_, 0, 256));
+    uint32_t* d_root = err_ + 1;
+    hipLaunchKernelGGL(indexingpu_amd::init_root_kernel, dim3(1), dim3(64), 0, 0, allocator_, err_,
+                       d_root);
+    IGPU_HIP_TRY(hipGetLastError());
+    IGPU_HIP_TRY(hipDeviceSynchronize());
+    IGPU_HIP_TRY(hipMemcpy(&root_index_, d_root, 4, hipMemcpyDeviceToHost));
+    if (root_index_ == invalid_pointer)
+      throw indexingpu_amd::hip_error("gpu_masstree: slab pool exhausted at construction");
+  }
+
+  bool owner_ = true;
+  uint32_t* scan_stack_ = nullptr;
+  uint64_t scan_stack_words_ = 0;
+};
+
+}  // namespace GpuMasstree

@@ -1,0 +1,471 @@
+// Kernels of the MI355X Masstree.
+//
+//  * batch_kernel: persistent grid; every tile owns one request at a time and calls
+//    op_step() in one flat loop.  A tile that finishes its request immediately loads
+//    its next one (static stride over the batch), so tiles of a wave stay busy and a
+//    wave keeps up to 64/T independent node loads in flight per iteration.  Replaces
+//    the reference's batch_kernel + device funcs (include/kernels.hpp:31-79,116-456).
+//  * scan_kernel: one range query per tile at a time (gpu_masstree.hpp:250-395).
+//  * drain_kernel / reset_kernel: free the records retired by the previous launch.
+//  * init_root_kernel: allocates and writes the empty layer-0 root.
+#pragma once
+#include "hip_util.hpp"
+#include "layout.hpp"
+#include "reclaim.hpp"
+#include "slab_alloc.hpp"
+#include "tile_policy.hpp"
+#include "tree_ops.hpp"
+
+namespace indexingpu_amd {
+
+constexpr u32 kBlockThreads = 256;
+
+struct tree_view {
+  slab_pool_view pool;
+  retire_view retire;
+  u32* err;
+  u32 root;
+};
+
+// Arguments of one batch.  `types` selects per-request kinds (mixed batches); when it
+// is null every request has `kind`.  Find requests write min(len, maxV) value slices
+// at out_values + i * maxV and their length to out_value_lengths[i]; write requests
+// report success in results[i] when results is not null.
+struct batch_args {
+  const uint8_t* types;
+  u32 kind;
+  const u32* keys;
+  u32 max_key_length;
+  const u32* key_lengths;
+  const u32* values;
+  u32 max_value_length;
+  const u32* value_lengths;
+  u32* out_values;
+  u32* out_value_lengths;
+  bool* results;
+  u64 n;
+};
+
+template <u32 T>
+__device__ __forceinline__ void load_request(const batch_args& a, u64 i, request& rq) {
+  rq.kind = a.types ? (u32)a.types[i] : a.kind;
+  rq.key_length = a.key_lengths ? a.key_lengths[i] : a.max_key_length;
+  rq.key = a.keys + i * (u64)a.max_key_length;
+  rq.value = a.values ? a.values + i * (u64)a.max_value_length : nullptr;
+  rq.value_length = a.value_lengths ? a.value_lengths[i] : a.max_value_length;
+  rq.out_value = a.out_values ? a.out_values + i * (u64)a.max_value_length : nullptr;
+  rq.max_value_length = a.out_values ? a.max_value_length : 0u;
+}
+
+// After a corruption or livelock guard fired, the remaining requests of the launch are
+// skipped so that the kernel drains quickly.
+__device__ __forceinline__ bool abort_pending(u32* err) {
+  const u32 e = dev::load_relaxed(err);
+  return e == kErrStepLimit || e == kErrBadIndex;
+}
+
+template <u32 T, bool Validate>
+__global__ void __launch_bounds__(kBlockThreads) batch_kernel(tree_view tv, batch_args a) {
+  wave_tile<T> c(tv.pool, tv.retire, tv.err);
+  const u32 tiles_per_block = kBlockThreads / T;
+  const u64 ntiles = (u64)gridDim.x * tiles_per_block;
+  u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
+  op_state<gpu_node> st;
+  request rq;
+  bool active = i < a.n;
+  auto start = [&]() {
+    load_request<T>(a, i, rq);
+    const bool writes = rq.kind == op_insert || rq.kind == op_update || rq.kind == op_upsert;
+    const bool bad_len = rq.key_length == 0 || rq.key_length > a.max_key_length ||
+                         (writes && rq.value_length > a.max_value_length);
+    if (bad_len) rq.key_length = 0;  // nothing of the request is read
+    c.load_key_cache(rq);
+    op_begin(c, st, tv.root);
+    if (bad_len) {
+      c.set_error(kErrBadLength);
+      op_finish(st, 0);
+    } else if (abort_pending(tv.err)) {
+      op_finish(st, 0);
+    }
+  };
+  if (active) start();
+  while (active) {
+    bool progressed = true;
+    if (st.phase != ph_done) progressed = op_step<Validate>(c, rq, st);
+    if (st.phase == ph_done) {
+      if (c.lane == 0) {
+        if (rq.kind == op_find) {
+          if (a.out_value_lengths) a.out_value_lengths[i] = st.result;
+        } else if (a.results) {
+          a.results[i] = st.result != 0;
+        }
+      }
+      i += ntiles;
+      active = i < a.n;
+      if (active) start();
+    }
+    // every tile of this wave is waiting on a lock: back off briefly
+    if (__ballot(!progressed) == __ballot(true)) __builtin_amdgcn_s_sleep(2);
+  }
+  c.flush();
+}
+
+// ---------------------------------------------------------------------------------
+// Range scan (read-only).  Outputs per query q at slot q * max_count + c: values
+// (max_value_length slices), value lengths, keys (max_key_length slices), key
+// lengths; counts[q].  Semantics follow the reference's cooperative_scan: ascending
+// keys k with lower <= k (inclusive) and, with upper keys, k <= upper (inclusive).
+// Per tile, `stack` holds 3 words per trie layer: (slice, border node, upper-bound
+// flag) of every LINK entry on the current path; the key prefix is its slices.
+struct scan_args {
+  const u32* lower_keys;
+  const u32* lower_key_lengths;
+  u32 max_key_length;
+  u32 max_count;
+  u64 num_queries;
+  const u32* upper_keys;
+  const u32* upper_key_lengths;
+  u32* counts;
+  u32* values;
+  u32 max_value_length;
+  u32* value_lengths;
+  u32* out_keys;
+  u32* out_key_lengths;
+  const u32* limits;  // optional per-query limit (<= max_count)
+  u32* stack;         // per-tile layer stack, stack_layers * 3 words per tile
+  u32 stack_layers;
+};
+
+template <u32 T>
+struct scan_query {
+  const u32* lo;
+  u32 nlo;
+  const u32* hi;
+  u32 nhi;
+  bool use_hi;
+  u32 limit;
+  u32* vals;
+  u32* vlens;
+  u32* keys;
+  u32* klens;
+  u32 maxV;
+  u32 maxK;
+  u32* stack;
+  u32 stack_layers;
+};
+
+// number of entries e with (e.slice, e.more) <= (s, m)
+template <u32 T>
+__device__ __forceinline__ u32 entries_le(const wave_tile<T>& c, const gpu_node& n, u32 s, bool m) {
+  const u32 nk = c.nkeys_of(n);
+  const u32 myks = ks_get(c.ksbits(n), c.slot);
+  return (u32)__builtin_popcount(
+      c.ballot(c.slot < nk && !entry_less(s, m, slot_lo(n.s), ks_more(myks))));
+}
+
+// compares record tail (record head loaded in c) with key[off, off+len): -1, 0, +1
+template <u32 T>
+__device__ __forceinline__ int rec_tail_cmp(wave_tile<T>& c, const u32* key, u32 off, u32 len,
+                                            u32 len_word) {
+  const u32 tl = rec_tail_len(len_word);
+  const u32 common = tl < len ? tl : len;
+  const u32 need = 1 + common;
+  u64 cur = c.rec_slot;
+  for (u32 base = 0;; base += kRecordWordsPerSlab) {
+    if (base >= need) break;
+    const u32 d0 = base + 2 * c.slot, d1 = d0 + 1;
+    const bool v0 = d0 >= 1 && d0 < need;
+    const bool v1 = (2 * c.slot + 1 < kRecordWordsPerSlab) && d1 >= 1 && d1 < need;
+    const u32 k0 = v0 ? key[off + d0 - 1] : 0u;
+    const u32 k1 = v1 ? key[off + d1 - 1] : 0u;
+    const bool ne0 = v0 && k0 != slot_lo(cur);
+    const bool ne1 = v1 && k1 != slot_hi(cur);
+    // first differing word index: word order is lane0.lo, lane0.hi, lane1.lo, ...
+    const u32 m0 = c.ballot(ne0), m1 = c.ballot(ne1);
+    if (m0 | m1) {
+      const u32 l0 = m0 ? (u32)__builtin_ctz(m0) : 99u;
+      const u32 l1 = m1 ? (u32)__builtin_ctz(m1) : 99u;
+      const bool use_lo = l0 <= l1;
+      const u32 ln = use_lo ? l0 : l1;
+      const u32 rw = use_lo ? c.bcast(slot_lo(cur), ln) : c.bcast(slot_hi(cur), ln);
+      const u32 kw = use_lo ? c.bcast(k0, ln) : c.bcast(k1, ln);
+      return rw < kw ? -1 : 1;
+    }
+    if (base + kRecordWordsPerSlab >= need) break;
+    if (!c.rec_next(cur)) return 0;
+  }
+  return tl < len ? -1 : (tl > len ? 1 : 0);
+}
+
+// writes the output key for an entry: stack prefix (layers 0..l-1), slice, record tail
+template <u32 T>
+__device__ __forceinline__ void scan_write_key_prefix(wave_tile<T>& c, const scan_query<T>& q,
+                                                      u32 out, u32 l, u32 slice) {
+  if (!q.keys) return;
+  u32* dst = q.keys + (u64)out * q.maxK;
+  for (u32 j = c.lane; j <= l && j < q.maxK; j += T) {
+    dst[j] = (j == l) ? slice : q.stack[3 * j];
+  }
+}
+
+template <u32 T, bool Validate>
+__device__ __forceinline__ void scan_read(wave_tile<T>& c, u32 idx, gpu_node& n) {
+  for (u32 spin = 0; spin < (1u << 20); spin++) {
+    if (c.template read_node<Validate>(idx, n)) return;
+    if (c.failed()) return;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  c.set_error(kErrStepLimit);
+  c.fatal = true;
+}
+
+// Returns the number of results.
+template <u32 T, bool Validate>
+__device__ u32 tile_scan(wave_tile<T>& c, u32 root, const scan_query<T>& q) {
+  if (q.limit == 0) return 0;
+  u32 count = 0;
+  u32 l = 0;
+  bool lo = true;       // lower bound constrains this layer
+  bool hi = q.use_hi;   // upper bound constrains this layer
+  bool resume = false;  // continue after (rs, more) in the stacked border node
+  u32 rs = 0;
+  u32 node = root;
+  gpu_node n;
+  for (u32 guard = 0; guard < (1u << 20); guard++) {
+    if (c.failed()) return count;
+    // ---- position on the border node that holds the start of this layer's range
+    const u32 target = resume ? rs : (lo ? q.lo[l] : 0u);
+    while (true) {
+      scan_read<T, Validate>(c, node, n);
+      if (c.failed()) return count;
+      const u32 h = c.header(n);
+      if (!hdr_border(h)) {
+        u32 child = c.find_child(n, target);
+        node = (child != kInvalid) ? child : c.sibling(n);
+        continue;
+      }
+      if (target > c.high(n) && c.sibling(n) != kInvalid) {
+        node = c.sibling(n);
+        continue;
+      }
+      break;
+    }
+    const bool lmore = l + 1 < q.nlo;
+    const bool umore = l + 1 < q.nhi;
+    const u32 uslice = hi ? q.hi[l] : 0u;
+    u32 p;
+    if (resume) p = entries_le(c, n, rs, true);
+    else if (lo) p = c.insert_position(n, q.lo[l], lmore);
+    else p = 0;
+    bool first_node = true;
+    bool descended = false;
+    while (true) {
+      const u32 nk = c.nkeys_of(n);
+      const u32 ksb = c.ksbits(n);
+      const u32 end = hi ? min(nk, entries_le(c, n, uslice, umore)) : nk;
+      const bool hi_stop = hi && (end < nk || c.high(n) >= uslice);
+      // lower-bound check of a suffix entry equal to (lo[l], more)
+      if (lo && !resume && first_node && p < end && lmore) {
+        const u32 s0 = c.bcast(slot_lo(n.s), p);
+        const u32 k0 = ks_get(ksb, p);
+        if (s0 == q.lo[l] && k0 == kSuffix) {
+          u32 lw;
+          if (!c.rec_load(c.bcast(slot_hi(n.s), p), lw)) return count;
+          if (rec_tail_cmp(c, q.lo, l + 1, q.nlo - l - 1, lw) < 0) p += 1;
+        }
+      }
+      // first link entry in [p, end)
+      const u32 myks = ks_get(ksb, c.slot);
+      const u32 links = c.ballot(c.slot >= p && c.slot < end && myks == kLink);
+      const u32 lk = links ? (u32)__builtin_ctz(links) : end;
+      u32 out_end = lk;
+      bool stop_after = false;
+      // upper-bound check of a suffix entry equal to (hi[l], true)
+      if (hi && umore && out_end > p) {
+        const u32 e = out_end - 1;
+        const u32 se = c.bcast(slot_lo(n.s), e);
+        if (se == uslice && ks_get(ksb, e) == kSuffix) {
+          u32 lw;
+          if (!c.rec_load(c.bcast(slot_hi(n.s), e), lw)) return count;
+          if (rec_tail_cmp(c, q.hi, l + 1, q.nhi - l - 1, lw) > 0) {
+            out_end = e;
+            stop_after = true;
+          }
+        }
+      }
+      // emit entries [p, out_end), at most limit - count
+      u32 take = out_end > p ? out_end - p : 0u;
+      if (take > q.limit - count) take = q.limit - count;
+      for (u32 e = p; e < p + take; e++) {
+        const u32 o = count + (e - p);
+        const u32 slice = c.bcast(slot_lo(n.s), e);
+        const u32 val = c.bcast(slot_hi(n.s), e);
+        const u32 ks = ks_get(ksb, e);
+        scan_write_key_prefix(c, q, o, l, slice);
+        if (ks == kValue) {
+          if (c.lane == 0) {
+            if (q.vlens) q.vlens[o] = 1;
+            if (q.klens) q.klens[o] = l + 1;
+            if (q.vals && q.maxV >= 1) q.vals[(u64)o * q.maxV] = val;
+          }
+        } else {
+          u32 lw;
+          if (!c.rec_load(val, lw)) return count;
+          const u32 tl = rec_tail_len(lw), vl = rec_value_len(lw);
+          if (c.lane == 0) {
+            if (q.vlens) q.vlens[o] = vl;
+            if (q.klens) q.klens[o] = l + 1 + tl;
+          }
+          // record words: tail -> key slices l+1.., value -> value slots
+          const u32 total = 1 + tl + vl;
+          u64 cur = c.rec_slot;
+          for (u32 base = 0; base < total; base += kRecordWordsPerSlab) {
+            if (c.lane < 16) {
+              for (u32 hh = 0; hh < 2; hh++) {
+                const u32 w = 2 * c.slot + hh;
+                if (w >= kRecordWordsPerSlab) continue;
+                const u32 d = base + w;
+                if (d == 0 || d >= total) continue;
+                const u32 word = hh ? slot_hi(cur) : slot_lo(cur);
+                if (d <= tl) {
+                  const u32 kk = l + d;
+                  if (q.keys && kk < q.maxK) q.keys[(u64)o * q.maxK + kk] = word;
+                } else {
+                  const u32 vv = d - 1 - tl;
+                  if (q.vals && vv < q.maxV) q.vals[(u64)o * q.maxV + vv] = word;
+                }
+              }
+            }
+            if (base + kRecordWordsPerSlab >= total) break;
+            if (!c.rec_next(cur)) return count;
+          }
+        }
+      }
+      count += take;
+      if (count >= q.limit || stop_after) return count;
+      if (lk < end) {
+        // descend into the next layer
+        if (l + 1 >= q.stack_layers) {
+          c.set_error(kErrScanScratch);
+          return count;
+        }
+        const u32 slice = c.bcast(slot_lo(n.s), lk);
+        const u32 child = c.bcast(slot_hi(n.s), lk);
+        if (c.lane == 0) {
+          q.stack[3 * l] = slice;
+          q.stack[3 * l + 1] = node;
+          q.stack[3 * l + 2] = hi ? 1u : 0u;
+        }
+        // other lanes of the tile read the stack later (key prefixes, pops)
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        const bool at_lo = lo && !resume && first_node && lk == p && slice == q.lo[l] && lmore;
+        lo = at_lo;
+        hi = hi && slice == uslice && umore;
+        l += 1;
+        node = child;
+        resume = false;
+        descended = true;
+        break;
+      }
+      if (hi_stop) return count;
+      const u32 sib = c.sibling(n);
+      if (sib == kInvalid) break;  // layer exhausted
+      node = sib;
+      scan_read<T, Validate>(c, node, n);
+      if (c.failed()) return count;
+      p = 0;
+      first_node = false;
+      lo = false;
+      resume = false;
+    }
+    if (descended) continue;
+    // pop to the previous layer
+    if (l == 0) return count;
+    l -= 1;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    rs = q.stack[3 * l];
+    node = q.stack[3 * l + 1];
+    hi = q.stack[3 * l + 2] != 0;
+    lo = false;
+    resume = true;
+  }
+  c.set_error(kErrStepLimit);
+  return count;
+}
+
+template <u32 T, bool Validate>
+__global__ void __launch_bounds__(kBlockThreads) scan_kernel(tree_view tv, scan_args a) {
+  wave_tile<T> c(tv.pool, tv.retire, tv.err);
+  const u32 tiles_per_block = kBlockThreads / T;
+  const u64 ntiles = (u64)gridDim.x * tiles_per_block;
+  const u64 gtile = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
+  for (u64 i = gtile; i < a.num_queries; i += ntiles) {
+    scan_query<T> q;
+    q.nlo = a.lower_key_lengths ? a.lower_key_lengths[i] : a.max_key_length;
+    q.lo = a.lower_keys + i * (u64)a.max_key_length;
+    q.use_hi = a.upper_keys != nullptr;
+    q.hi = q.use_hi ? a.upper_keys + i * (u64)a.max_key_length : nullptr;
+    q.nhi = q.use_hi ? (a.upper_key_lengths ? a.upper_key_lengths[i] : a.max_key_length) : 0u;
+    u32 lim = a.max_count;
+    if (a.limits && a.limits[i] < lim) lim = a.limits[i];
+    q.limit = lim;
+    const u64 slot0 = i * (u64)a.max_count;
+    q.vals = a.values ? a.values + slot0 * a.max_value_length : nullptr;
+    q.vlens = a.value_lengths ? a.value_lengths + slot0 : nullptr;
+    q.keys = a.out_keys ? a.out_keys + slot0 * a.max_key_length : nullptr;
+    q.klens = a.out_key_lengths ? a.out_key_lengths + slot0 : nullptr;
+    q.maxV = a.max_value_length;
+    q.maxK = a.max_key_length;
+    q.stack = a.stack + gtile * (u64)a.stack_layers * 3u;
+    q.stack_layers = a.stack_layers;
+    u32 cnt = 0;
+    if (q.nlo == 0 || q.nlo > a.max_key_length || (q.use_hi && (q.nhi == 0 || q.nhi > a.max_key_length))) {
+      c.set_error(kErrBadLength);
+    } else {
+      cnt = tile_scan<T, Validate>(c, tv.root, q);
+    }
+    if (c.lane == 0 && a.counts) a.counts[i] = cnt;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+__global__ void drain_kernel(slab_pool_view pool, retire_view rv) {
+  const u32 raw = *rv.count;
+  const u32 count = raw < rv.capacity ? raw : rv.capacity;
+  const u32 stride = gridDim.x * blockDim.x;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const u32 r = rv.list[i];
+    if (r >= pool.num_slabs) continue;
+    const u32* w = reinterpret_cast<const u32*>(pool.pool + (u64)r * kSlots);
+    const u32 lw = w[0];
+    const u32 ns = rec_num_slabs(1 + rec_tail_len(lw) + rec_value_len(lw));
+    u32 cur = r;
+    for (u32 s = 0; s < ns && cur < pool.num_slabs; s++) {
+      const u32 nxt = reinterpret_cast<const u32*>(pool.pool + (u64)cur * kSlots)[kRecordNextWord];
+      dev::free_slab(pool, cur);
+      cur = nxt;
+    }
+  }
+}
+
+__global__ void reset_kernel(retire_view rv) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *rv.count = 0;
+}
+
+__global__ void init_root_kernel(slab_pool_view pool, u32* err, u32* out_root) {
+  if (threadIdx.x >= 16) return;
+  wave_tile<16> c(pool, retire_view{}, err);
+  const u32 r = c.alloc_slab();
+  if (r != kInvalid) {
+    gpu_node n;
+    n.s = 0;
+    if (c.slot == kHighSlot) n.s = slot_make(kMaxSlice, 0);
+    if (c.slot == kMetaSlot) n.s = slot_make(hdr_make(0, true, true, 0), kInvalid);
+    c.write_node(r, n);
+  }
+  c.flush();
+  if (c.lane == 0) *out_root = r;
+}
+
+// Copies the pool's slabs [first, first + count) for the host walker (debug only).
+}  // namespace indexingpu_amd

@@ -1,0 +1,531 @@
+// gfx950 tile policy: the node, record and slab primitives of the Masstree state
+// machine (tree_ops.hpp), written for wave64.
+//
+// A tile is T = 16, 32 or 64 consecutive lanes of a wave.  Lane l holds node slot
+// (l & 15); with T > 16 the extra lanes mirror the first 16 (their loads coalesce
+// into the same 128-byte line, their stores are skipped).  Searches are one 64-bit
+// wave ballot masked to the tile's 16 slot bits plus one ds_bpermute broadcast;
+// inserts, removals and splits are lane-parallel shuffles (ds_bpermute) followed by
+// one coalesced store of the whole node.
+//
+// Memory ordering (concurrent batches): node slots are read with agent-scope relaxed
+// atomic loads (one 128-byte line, one instruction), the header word is re-read after
+// an agent-scope acquire fence and must match the unlocked header seen in the line.
+// Writers lock the header with an acquire fetch_or, write the node with relaxed
+// stores, and release the lock with a release fetch_add that also advances the
+// version.  Read-only batches (find<concurrent=false>, scan) use plain cached loads,
+// as the reference's weak memory order does (include/gpu_masstree.hpp:206).
+#pragma once
+#include "hip_util.hpp"
+#include "layout.hpp"
+#include "reclaim.hpp"
+#include "slab_alloc.hpp"
+#include "tree_ops.hpp"
+
+namespace indexingpu_amd {
+
+struct gpu_node {
+  u64 s;  // this lane's slot
+};
+
+template <u32 T>
+struct wave_tile {
+  static_assert(T == 16 || T == 32 || T == 64, "tile width must be 16, 32 or 64 lanes");
+  using node_t = gpu_node;
+
+  slab_pool_view pool;
+  retire_view rv;
+  u32* err;
+  u32 lane;       // lane within the tile
+  u32 slot;       // node slot owned by this lane
+  u32 shift;      // position of the tile inside the wave ballot
+  u32 key_cache;  // key[lane] of the current request
+  u64 rec_slot;   // this lane's 8 bytes of the loaded record head slab
+  u32 rec_idx;
+  bool fatal;
+  tile_slab_cache ac;
+  tile_retire_cache rc;
+
+  __device__ __forceinline__ wave_tile(const slab_pool_view& p, const retire_view& r, u32* e)
+      : pool(p), rv(r), err(e) {
+    const u32 l64 = __lane_id();
+    lane = l64 & (T - 1);
+    slot = lane & 15u;
+    shift = l64 & ~(T - 1);
+    key_cache = 0;
+    rec_slot = 0;
+    rec_idx = kInvalid;
+    fatal = false;
+  }
+
+  // ---------------- tile collectives ----------------
+  __device__ __forceinline__ u32 ballot(bool p) const {
+    return (u32)(__ballot(p) >> shift) & 0xFFFFu;
+  }
+  __device__ __forceinline__ u32 bcast(u32 v, u32 src) const { return __shfl(v, (int)src, (int)T); }
+  __device__ __forceinline__ u64 bcast64(u64 v, u32 src) const {
+    return __shfl(v, (int)src, (int)T);
+  }
+  // lane of slot s in this lane's mirror group
+  __device__ __forceinline__ u32 src_lane(u32 s) const { return (lane & ~15u) + s; }
+  __device__ __forceinline__ u64 slot_from(u64 v, u32 s) const {
+    return __shfl(v, (int)src_lane(s), (int)T);
+  }
+
+  __device__ __forceinline__ void set_error(u32 code) {
+    if (lane == 0) dev::set_error(err, code);
+  }
+  __device__ __forceinline__ bool failed() const { return fatal; }
+  __device__ __forceinline__ bool check(u32 idx) {
+    if (idx < pool.num_slabs) return true;
+    set_error(kErrBadIndex);
+    fatal = true;
+    return false;
+  }
+  __device__ __forceinline__ u64* slab(u32 idx) const { return pool.pool + (u64)idx * kSlots; }
+  __device__ __forceinline__ u32* hdr_ptr(u32 idx) const {
+    return reinterpret_cast<u32*>(slab(idx) + kMetaSlot);
+  }
+
+  // ---------------- node reads / locking ----------------
+  template <bool Validate>
+  __device__ __forceinline__ bool read_node(u32 idx, node_t& n) {
+    if (!check(idx)) return false;
+    const u64* p = slab(idx) + slot;
+    if (!Validate) {
+      n.s = *p;
+      return true;
+    }
+    n.s = dev::load_relaxed(p);
+    const u32 h = bcast(slot_lo(n.s), kMetaSlot);
+    if (hdr_locked(h)) return false;
+    dev::fence_acquire();
+    const u32 h2 = dev::load_relaxed(hdr_ptr(idx));
+    return h2 == h;
+  }
+  __device__ __forceinline__ void read_node_locked(u32 idx, node_t& n) {
+    n.s = dev::load_relaxed(slab(idx) + slot);
+  }
+  __device__ __forceinline__ bool try_lock(u32 idx, u32& old) {
+    if (!check(idx)) return false;
+    u32 o = 0;
+    if (lane == 0) o = dev::atomic_or_acq(hdr_ptr(idx), kLockBit);
+    old = bcast(o, 0);
+    if (hdr_locked(old)) return false;
+    dev::fence_release();
+    return true;
+  }
+  __device__ __forceinline__ void unlock_same(u32 idx) {
+    if (lane == 0) dev::atomic_and_rel(hdr_ptr(idx), ~kLockBit);
+  }
+  __device__ __forceinline__ void unlock_bump(u32 idx) {
+    if (lane == 0) dev::atomic_add_rel(hdr_ptr(idx), kVersionOne - kLockBit);
+  }
+  __device__ __forceinline__ void write_node(u32 idx, const node_t& n) {
+    if (lane < 16) dev::store_relaxed(slab(idx) + slot, n.s);
+  }
+  __device__ __forceinline__ void fence_release() { dev::fence_release(); }
+
+  // ---------------- node queries ----------------
+  __device__ __forceinline__ u32 header(const node_t& n) const {
+    return bcast(slot_lo(n.s), kMetaSlot);
+  }
+  __device__ __forceinline__ u32 nkeys_of(const node_t& n) const { return hdr_nkeys(header(n)); }
+  __device__ __forceinline__ u32 sibling(const node_t& n) const {
+    return bcast(slot_hi(n.s), kMetaSlot);
+  }
+  __device__ __forceinline__ u32 high(const node_t& n) const { return bcast(slot_lo(n.s), kHighSlot); }
+  __device__ __forceinline__ u32 ksbits(const node_t& n) const {
+    return bcast(slot_hi(n.s), kHighSlot);
+  }
+  __device__ __forceinline__ void set_header(node_t& n, u32 h) const {
+    if (slot == kMetaSlot) n.s = slot_make(h, slot_hi(n.s));
+  }
+  __device__ __forceinline__ void set_sibling(node_t& n, u32 s) const {
+    if (slot == kMetaSlot) n.s = slot_make(slot_lo(n.s), s);
+  }
+  __device__ __forceinline__ void set_high(node_t& n, u32 hk) const {
+    if (slot == kHighSlot) n.s = slot_make(hk, slot_hi(n.s));
+  }
+
+  __device__ __forceinline__ u32 find_child(const node_t& n, u32 q) const {
+    const u32 nk = nkeys_of(n);
+    const u32 m = ballot(slot < nk && q <= slot_lo(n.s));
+    const u32 child = bcast(slot_hi(n.s), m ? (u32)__builtin_ctz(m) : 0u);
+    return m ? child : kInvalid;
+  }
+  __device__ __forceinline__ bool find_child_slot(const node_t& n, u32 child, u32& j) const {
+    const u32 nk = nkeys_of(n);
+    const u32 m = ballot(slot < nk && slot_hi(n.s) == child);
+    j = m ? (u32)__builtin_ctz(m) : 0u;
+    return m != 0;
+  }
+  __device__ __forceinline__ int lookup(const node_t& n, u32 q, bool more, u32& ks,
+                                        u32& val) const {
+    const u32 nk = nkeys_of(n);
+    const u32 b = ksbits(n);
+    const u32 myks = ks_get(b, slot);
+    const u32 m = ballot(slot < nk && slot_lo(n.s) == q && ks_more(myks) == more);
+    const u32 pos = m ? (u32)__builtin_ctz(m) : 0u;
+    const u32 v = bcast(slot_hi(n.s), pos);
+    if (!m) return -1;
+    ks = ks_get(b, pos);
+    val = v;
+    return (int)pos;
+  }
+  __device__ __forceinline__ u32 insert_position(const node_t& n, u32 q, bool more) const {
+    const u32 nk = nkeys_of(n);
+    const u32 myks = ks_get(ksbits(n), slot);
+    return (u32)__builtin_popcount(
+        ballot(slot < nk && entry_less(slot_lo(n.s), ks_more(myks), q, more)));
+  }
+
+  // ---------------- node edits ----------------
+  __device__ __forceinline__ void border_insert(node_t& n, u32 pos, u32 q, u32 ks, u32 val) const {
+    const u32 h = header(n), b = ksbits(n), nk = hdr_nkeys(h);
+    const u64 prev = slot_from(n.s, slot > 0 ? slot - 1 : 0);
+    const u32 lowmask = pos == 0 ? 0u : ((1u << (2 * pos)) - 1u);
+    const u32 nb = (b & lowmask) | ((b & ~lowmask) << 2) | (ks << (2 * pos));
+    if (slot < kBorderCap) {
+      if (slot == pos) n.s = slot_make(q, val);
+      else if (slot > pos && slot <= nk) n.s = prev;
+    } else if (slot == kHighSlot) {
+      n.s = slot_make(slot_lo(n.s), nb);
+    } else {
+      n.s = slot_make(hdr_with_nkeys(h, nk + 1), slot_hi(n.s));
+    }
+  }
+  __device__ __forceinline__ void border_remove(node_t& n, u32 pos) const {
+    const u32 h = header(n), b = ksbits(n), nk = hdr_nkeys(h);
+    const u64 next = slot_from(n.s, slot < 15 ? slot + 1 : 15);
+    const u32 lowmask = pos == 0 ? 0u : ((1u << (2 * pos)) - 1u);
+    const u32 highpart = (b >> (2 * (pos + 1))) << (2 * pos);
+    if (slot < kBorderCap) {
+      if (slot >= pos) {
+        if (slot + 1 < nk) n.s = next;
+        else if (slot + 1 == nk) n.s = 0;
+      }
+    } else if (slot == kHighSlot) {
+      n.s = slot_make(slot_lo(n.s), (b & lowmask) | highpart);
+    } else {
+      n.s = slot_make(hdr_with_nkeys(h, nk - 1), slot_hi(n.s));
+    }
+  }
+  __device__ __forceinline__ void border_set(node_t& n, u32 pos, u32 ks, u32 val) const {
+    const u32 b = ksbits(n);
+    if (slot == pos) n.s = slot_make(slot_lo(n.s), val);
+    if (slot == kHighSlot)
+      n.s = slot_make(slot_lo(n.s), (b & ~(3u << (2 * pos))) | (ks << (2 * pos)));
+  }
+  // Splits n's entries plus the pending (q, ks, val) at ipos into two nodes; returns
+  // the separator (largest slice of the left node).  Entries with one slice stay
+  // together.  Sets only nkeys in the result headers.
+  __device__ __forceinline__ u32 border_split(const node_t& n, u32 ipos, u32 q, u32 ks, u32 val,
+                                              node_t& left, node_t& right) const {
+    const u32 b = ksbits(n), nk = nkeys_of(n), total = nk + 1;
+    u32 m = (total + 1) / 2;
+    {
+      const u32 i0 = m - 1, i1 = m;
+      const u32 s0 = bcast(slot_lo(n.s), i0 < ipos ? i0 : (i0 > ipos ? i0 - 1 : 0));
+      const u32 s1 = bcast(slot_lo(n.s), i1 < ipos ? i1 : (i1 > ipos ? i1 - 1 : 0));
+      const u32 v0 = (i0 == ipos) ? q : s0;
+      const u32 v1 = (i1 == ipos) ? q : s1;
+      if (v0 == v1) m -= 1;
+    }
+    const u32 li = slot, ri = m + slot;
+    const u32 lsrc = li < ipos ? li : (li > ipos ? li - 1 : 0);
+    u32 rsrc = ri < ipos ? ri : (ri > ipos ? ri - 1 : 0);
+    rsrc = rsrc > 15 ? 15 : rsrc;
+    u64 lv = slot_from(n.s, lsrc);
+    u64 rv = slot_from(n.s, rsrc);
+    if (li == ipos) lv = slot_make(q, val);
+    if (ri == ipos) rv = slot_make(q, val);
+    u32 lb = 0, rb = 0;
+    for (u32 i = 0; i < total; i++) {
+      const u32 k = i < ipos ? ks_get(b, i) : (i == ipos ? ks : ks_get(b, i - 1));
+      if (i < m) lb |= k << (2 * i);
+      else rb |= k << (2 * (i - m));
+    }
+    left.s = (slot < m) ? lv : 0;
+    right.s = (slot < total - m) ? rv : 0;
+    if (slot == kHighSlot) {
+      left.s = slot_make(0, lb);
+      right.s = slot_make(0, rb);
+    } else if (slot == kMetaSlot) {
+      left.s = slot_make(hdr_with_nkeys(0, m), kInvalid);
+      right.s = slot_make(hdr_with_nkeys(0, total - m), kInvalid);
+    }
+    const u32 i0 = m - 1;
+    const u32 s0 = bcast(slot_lo(n.s), i0 < ipos ? i0 : (i0 > ipos ? i0 - 1 : 0));
+    return (i0 == ipos) ? q : s0;
+  }
+  __device__ __forceinline__ void make_interior_root(node_t& n, u32 sep, u32 l, u32 r, u32 level,
+                                                     u32 h) const {
+    const u32 keep = h & (kLockBit | ~((1u << kVersionShift) - 1u));
+    u64 v = 0;
+    if (slot == 0) v = slot_make(sep, l);
+    if (slot == 1) v = slot_make(kMaxSlice, r);
+    if (slot == kMetaSlot) v = slot_make(keep | hdr_make(2, false, true, level), kInvalid);
+    n.s = v;
+  }
+  __device__ __forceinline__ void make_single_root(node_t& x, u32 t0, u32 ks, u32 val) const {
+    u64 v = 0;
+    if (slot == 0) v = slot_make(t0, val);
+    if (slot == kHighSlot) v = slot_make(kMaxSlice, ks);
+    if (slot == kMetaSlot) v = slot_make(hdr_make(1, true, true, 0), kInvalid);
+    x.s = v;
+  }
+  __device__ __forceinline__ void interior_insert(node_t& n, u32 j, u32 sep, u32 lc, u32 rc) const {
+    const u32 h = header(n), nk = hdr_nkeys(h);
+    const u32 old_bound = bcast(slot_lo(n.s), j);
+    const u64 prev = slot_from(n.s, slot > 0 ? slot - 1 : 0);
+    if (slot < kInteriorCap) {
+      if (slot == j) n.s = slot_make(sep, lc);
+      else if (slot == j + 1) n.s = slot_make(old_bound, rc);
+      else if (slot > j + 1 && slot <= nk) n.s = prev;
+    } else {
+      n.s = slot_make(hdr_with_nkeys(h, nk + 1), slot_hi(n.s));
+    }
+  }
+  __device__ __forceinline__ u32 interior_split(const node_t& n, u32 j, u32 sep, u32 lc, u32 rc,
+                                                node_t& left, node_t& right) const {
+    const u32 nk = nkeys_of(n), total = nk + 1, m = total / 2;
+    const u32 bj = bcast(slot_lo(n.s), j);
+    const u32 li = slot, ri = m + slot;
+    auto src = [&](u32 i) -> u32 {
+      u32 s = i < j ? i : (i > j + 1 ? i - 1 : 0);
+      return s > 15 ? 15 : s;
+    };
+    u64 lv = slot_from(n.s, src(li));
+    u64 rv = slot_from(n.s, src(ri));
+    if (li == j) lv = slot_make(sep, lc);
+    if (li == j + 1) lv = slot_make(bj, rc);
+    if (ri == j) rv = slot_make(sep, lc);
+    if (ri == j + 1) rv = slot_make(bj, rc);
+    left.s = (slot < m) ? lv : 0;
+    right.s = (slot < total - m) ? rv : 0;
+    if (slot == kMetaSlot) {
+      left.s = slot_make(hdr_with_nkeys(0, m), kInvalid);
+      right.s = slot_make(hdr_with_nkeys(0, total - m), kInvalid);
+    }
+    const u32 i0 = m - 1;
+    const u32 s0 = bcast(slot_lo(n.s), i0 < j ? i0 : (i0 > j + 1 ? i0 - 1 : 0));
+    return i0 == j ? sep : (i0 == j + 1 ? bj : s0);
+  }
+
+  // ---------------- keys and values ----------------
+  __device__ __forceinline__ void load_key_cache(const request& rq) {
+    key_cache = (lane < rq.key_length) ? rq.key[lane] : 0u;
+  }
+  __device__ __forceinline__ u32 key_slice(const request& rq, u32 l) const {
+    return l < T ? bcast(key_cache, l) : rq.key[l];
+  }
+  // per-lane gather of key slice k (k < key length); every lane must call it
+  __device__ __forceinline__ u32 key_gather(const request& rq, u32 k) const {
+    const u32 c = __shfl(key_cache, (int)(k & (T - 1)), (int)T);
+    return k < T ? c : rq.key[k];
+  }
+  __device__ __forceinline__ u32 value_slice(const request& rq, u32 i) const { return rq.value[i]; }
+  __device__ __forceinline__ void write_inline_value(const request& rq, u32 v) const {
+    if (lane == 0 && rq.max_value_length >= 1) rq.out_value[0] = v;
+  }
+
+  // ---------------- slabs ----------------
+  __device__ __forceinline__ u32 alloc_slab() {
+    if (ac.bits == 0) {
+      u32 w, b;
+      const bool ok = dev::claim_word(pool, lane == 0, w, b);
+      w = bcast(w, 0);
+      b = bcast(b, 0);
+      const u32 okv = bcast((u32)ok, 0);
+      if (!okv) {
+        set_error(kErrOutOfMemory);
+        return kInvalid;
+      }
+      ac.word = w;
+      ac.bits = b;
+    }
+    const u32 j = (u32)__builtin_ctz(ac.bits);
+    ac.bits &= ac.bits - 1;
+    return ac.word * 32u + j;
+  }
+  __device__ __forceinline__ void flush() {
+    if (ac.bits != 0 && lane == 0) dev::atomic_and_relaxed(&pool.bitmap[ac.word], ~ac.bits);
+    ac.bits = 0;
+    if (rc.base != kInvalid) {
+      const u32 pad = kRetireChunk - rc.used;
+      if (lane < pad) rv.list[rc.base + rc.used + lane] = kInvalid;
+      rc.base = kInvalid;
+    }
+  }
+  __device__ __forceinline__ void retire(u32 idx) {
+    if (!rv.enabled) return;
+    if (rc.base == kInvalid || rc.used == kRetireChunk) {
+      u32 b = 0;
+      if (lane == 0) b = dev::atomic_add_relaxed(rv.count, kRetireChunk);
+      b = bcast(b, 0);
+      if (b + kRetireChunk > rv.capacity) {
+        if (lane == 0) dev::atomic_add_relaxed(rv.leaked, 1u);
+        rc.base = kInvalid;
+        return;
+      }
+      rc.base = b;
+      rc.used = 0;
+    }
+    if (lane == 0) rv.list[rc.base + rc.used] = idx;
+    rc.used += 1;
+  }
+
+  // ---------------- records ----------------
+  __device__ __forceinline__ bool rec_load(u32 idx, u32& len_word) {
+    if (!check(idx)) return false;
+    rec_idx = idx;
+    rec_slot = slab(idx)[slot];
+    len_word = bcast(slot_lo(rec_slot), 0);
+    return true;
+  }
+  __device__ __forceinline__ bool rec_next(u64& cur) {
+    const u32 nxt = bcast(slot_hi(cur), kMetaSlot);
+    if (!check(nxt)) return false;
+    cur = slab(nxt)[slot];
+    return true;
+  }
+  // compares the record tail with key[off, off + len); the record head is loaded
+  __device__ __forceinline__ bool rec_tail_equal(const u32* key, u32 off, u32 len, u32 len_word) {
+    if (rec_tail_len(len_word) != len) return false;
+    const u32 need = 1 + len;
+    request kr;
+    kr.key = key;
+    u64 cur = rec_slot;
+    for (u32 base = 0;; base += kRecordWordsPerSlab) {
+      const u32 w0 = 2 * slot, w1 = 2 * slot + 1;
+      const u32 d0 = base + w0, d1 = base + w1;
+      const u32 k0 = off + d0 - 1, k1 = off + d1 - 1;
+      const bool c0 = d0 >= 1 && d0 < need;
+      const bool c1 = w1 < kRecordWordsPerSlab && d1 >= 1 && d1 < need;
+      const u32 g0 = key_gather(kr, c0 ? k0 : 0u);
+      const u32 g1 = key_gather(kr, c1 ? k1 : 0u);
+      const bool mis = (c0 && g0 != slot_lo(cur)) || (c1 && g1 != slot_hi(cur));
+      if (ballot(mis)) return false;
+      if (base + kRecordWordsPerSlab >= need) return true;
+      if (!rec_next(cur)) return false;
+    }
+  }
+  __device__ __forceinline__ u32 rec_copy_value(const request& rq, u32 len_word) {
+    const u32 tl = rec_tail_len(len_word), vl = rec_value_len(len_word);
+    const u32 n = vl < rq.max_value_length ? vl : rq.max_value_length;
+    const u32 first = 1 + tl, end = first + n;
+    u64 cur = rec_slot;
+    for (u32 base = 0; base < end; base += kRecordWordsPerSlab) {
+      if (base + kRecordWordsPerSlab > first && lane < 16) {
+        const u32 d0 = base + 2 * slot, d1 = d0 + 1;
+        if (d0 >= first && d0 < end) rq.out_value[d0 - first] = slot_lo(cur);
+        if (2 * slot + 1 < kRecordWordsPerSlab && d1 >= first && d1 < end)
+          rq.out_value[d1 - first] = slot_hi(cur);
+      }
+      if (base + kRecordWordsPerSlab >= end) break;
+      if (!rec_next(cur)) break;
+    }
+    return vl;
+  }
+  // writes a new record whose data words come from word(d) (lane-parallel)
+  template <class WordFn>
+  __device__ __forceinline__ u32 rec_write(u32 total, WordFn word) {
+    const u32 nslabs = rec_num_slabs(total);
+    u32 cur = alloc_slab();
+    if (cur == kInvalid) return kInvalid;
+    const u32 head = cur;
+    for (u32 s = 0; s < nslabs; s++) {
+      u32 nxt = kInvalid;
+      if (s + 1 < nslabs) {
+        nxt = alloc_slab();
+        if (nxt == kInvalid) return kInvalid;
+      }
+      const u32 d0 = s * kRecordWordsPerSlab + 2 * slot, d1 = d0 + 1;
+      const u32 v0 = word(d0, s, 0);
+      const u32 v1 = (2 * slot + 1 == kRecordNextWord) ? nxt : word(d1, s, 1);
+      if (lane < 16) slab(cur)[slot] = slot_make(v0, v1);
+      cur = nxt;
+    }
+    return head;
+  }
+  __device__ __forceinline__ u32 rec_create(const u32* key, u32 off, u32 tail_len, const u32* value,
+                                            u32 vlen) {
+    if (tail_len > kMaxLength || vlen > kMaxLength) {
+      set_error(kErrBadLength);
+      return kInvalid;
+    }
+    const u32 total = 1 + tail_len + vlen;
+    const u32 lw = rec_len_word(tail_len, vlen);
+    return rec_write(total, [&](u32 d, u32, u32) -> u32 {
+      if (d == 0) return lw;
+      if (d <= tail_len) return key[off + d - 1];
+      if (d < total) return value[d - 1 - tail_len];
+      return 0u;
+    });
+  }
+  // new record = old record's data words shifted down by one (drops tail[0]);
+  // the old record head is loaded
+  __device__ __forceinline__ u32 rec_clone_shift(u32 new_len_word, u32 count, u32 old_total) {
+    u64 oa = rec_slot;
+    u64 ob = 0;
+    bool have_b = old_total > kRecordWordsPerSlab;
+    if (have_b) {
+      ob = oa;
+      if (!rec_next(ob)) return kInvalid;
+    }
+    const u32 total = 1 + count;
+    const u32 nslabs = rec_num_slabs(total);
+    u32 cur = alloc_slab();
+    if (cur == kInvalid) return kInvalid;
+    const u32 head = cur;
+    for (u32 s = 0; s < nslabs; s++) {
+      u32 nxt = kInvalid;
+      if (s + 1 < nslabs) {
+        nxt = alloc_slab();
+        if (nxt == kInvalid) return kInvalid;
+      }
+      // new word d' = 31 s + w takes old word d' + 1
+      const u64 right = slot_from(oa, slot < 15 ? slot + 1 : 15);
+      const u64 b0 = slot_from(ob, 0);
+      const u32 d0 = s * kRecordWordsPerSlab + 2 * slot, d1 = d0 + 1;
+      u32 v0 = (slot < 15) ? slot_hi(oa) : slot_lo(b0);
+      u32 v1 = (slot < 15) ? slot_lo(right) : nxt;
+      if (d0 == 0) v0 = new_len_word;
+      else if (d0 >= total) v0 = 0;
+      if (slot < 15 && d1 >= total) v1 = 0;
+      if (lane < 16) slab(cur)[slot] = slot_make(v0, v1);
+      cur = nxt;
+      // advance the old window by one slab
+      oa = ob;
+      const bool more_old = (s + 2) * kRecordWordsPerSlab < old_total;
+      if (more_old) {
+        if (!rec_next(ob)) return kInvalid;
+      } else {
+        ob = 0;
+      }
+    }
+    return head;
+  }
+  __device__ __forceinline__ bool rec_push_down(u32 rec, u32 len_word, u32& t0, u32& eks, u32& eval) {
+    (void)rec;
+    const u32 tl = rec_tail_len(len_word), vl = rec_value_len(len_word);
+    t0 = bcast(slot_hi(rec_slot), 0);  // data word 1
+    const u32 old_total = 1 + tl + vl;
+    if (tl == 1 && vl == 1) {
+      eks = kValue;
+      eval = bcast(slot_lo(rec_slot), 1);  // data word 2
+      return true;
+    }
+    if (tl == 1) {
+      eks = kLongVal;
+      eval = rec_clone_shift(rec_len_word(0, vl), vl, old_total);
+    } else {
+      eks = kSuffix;
+      eval = rec_clone_shift(rec_len_word(tl - 1, vl), tl - 1 + vl, old_total);
+    }
+    return eval != kInvalid;
+  }
+};
+
+}  // namespace indexingpu_amd

@@ -1,0 +1,540 @@
+// Per-tile operation state machine of the MI355X Masstree.
+//
+// One tile (16, 32 or 64 lanes of a wave64) owns one operation at a time.  Each call
+// of `step()` visits at most one node (one coalesced 128-byte load) and then either
+// advances the operation or leaves it where it is ("retry": the node was locked, an
+// optimistic read was torn, or a try-lock failed).  Nothing inside a step waits on
+// another tile.  This matters on CDNA: the tiles of one wave share a program counter,
+// so a tile spinning in an inner loop on a lock held by a sibling tile of the same
+// wave would never let that sibling run.  A batch kernel calls `step()` in one flat
+// loop, so every tile of a wave advances once per iteration and lock holders always
+// make progress.
+//
+// The machine is generic over a tile policy `C` (node loads and stores, lane-parallel
+// node edits, record access, slab allocation, retirement).  The product instantiates
+// the gfx950 wave policy (tile_policy.hpp); the tests also instantiate a sequential
+// CPU policy over the same memory layout to check interleavings and concurrency on
+// the host.
+//
+// Algorithm (behaviour follows the reference's cooperative_{find,insert,update,erase},
+// include/gpu_masstree.hpp:194-248, :397-627, :629-749, :751-971; the concurrency
+// scheme is this project's own):
+//   * B-link tree per layer with high keys and right-sibling links; readers move right
+//     when the slice is above a node's high key.
+//   * Writers lock only border nodes on the way down; a full border node is split
+//     bottom-up: the new right node is written locked, the left node stays locked
+//     until the parent has been updated (lock order child -> parent, left -> right,
+//     so no cycles), and a full parent splits the same way.  Layer roots split in
+//     place, so a layer's root index never changes.
+//   * A key tail that collides with an existing suffix record is pushed one layer
+//     down (a new layer root holding the old entry) and the insert continues there.
+//   * Erase removes the entry without merging nodes (no node is ever freed, so an
+//     optimistic reader can never follow a pointer into a recycled node); records are
+//     retired to the launch's reclaimer and freed after the launch.
+#pragma once
+#include "layout.hpp"
+
+namespace indexingpu_amd {
+
+// Operation kinds; the first four values are the reference's request_type
+// (include/macros.hpp:51-58).
+enum op_kind : u32 {
+  op_insert = 0,
+  op_erase = 1,
+  op_find = 2,
+  op_update = 3,
+  op_upsert = 4,  // insert<update_if_exists=true>
+};
+
+enum op_phase : u32 {
+  ph_descend = 0,         // optimistic descent in the current layer
+  ph_record = 1,          // find: read the record `node`
+  ph_lock = 2,            // lock border `node` and apply the write
+  ph_parent_descend = 3,  // split: find the level that must receive (sep -> left, right)
+  ph_parent_lock = 4,     // split: lock that parent and insert the separator
+  ph_done = 5,
+};
+
+// One request, uniform over the tile.
+struct request {
+  u32 kind;
+  u32 key_length;
+  const u32* key;
+  const u32* value;     // insert / update / upsert input
+  u32 value_length;
+  u32* out_value;       // find output (max_value_length slices)
+  u32 max_value_length;
+};
+
+template <class NodeImage>
+struct op_state {
+  u32 phase;
+  u32 layer;       // index of the key slice handled by the current layer
+  u32 node;        // node (or record) visited by the next step
+  u32 layer_root;  // root of the current layer
+  u32 snap_hdr;    // header of `img` when it was validated (kInvalid = must reload)
+  u32 a0, a1, a2, a3;  // split propagation: left, right, separator, parent level
+  u32 result;      // find: value length; writes: 1 = success
+  u32 steps;
+  NodeImage img;   // last validated image of `node`
+};
+
+constexpr u32 kMaxStepsPerOp = 1u << 20;
+
+template <class C, class S>
+IGPU_HD void op_begin(C& c, S& st, u32 root) {
+  st.phase = ph_done;
+  st.layer = 0;
+  st.node = root;
+  st.layer_root = root;
+  st.snap_hdr = kInvalid;
+  st.a0 = st.a1 = st.a2 = st.a3 = 0;
+  st.result = 0;
+  st.steps = 0;
+  st.phase = ph_descend;
+  (void)c;
+}
+
+template <class S>
+IGPU_HD void op_finish(S& st, u32 result) {
+  st.result = result;
+  st.phase = ph_done;
+}
+
+template <class S>
+IGPU_HD void op_next_layer(S& st, u32 root) {
+  st.layer += 1;
+  st.node = root;
+  st.layer_root = root;
+  st.snap_hdr = kInvalid;
+  st.phase = ph_descend;
+}
+
+// Builds the entry that stores `rq`'s value at a key that ends at this slice.
+template <class C>
+IGPU_HD bool make_value_entry(C& c, const request& rq, u32& ks, u32& val) {
+  if (rq.value_length == 1) {
+    ks = kValue;
+    val = c.value_slice(rq, 0);
+    return true;
+  }
+  ks = kLongVal;
+  val = c.rec_create(rq.key, 0, 0, rq.value, rq.value_length);
+  return val != kInvalid;
+}
+
+// Returns true when the step made progress (false: retry later).
+template <bool Validate, class C, class S>
+IGPU_HD bool op_step(C& c, const request& rq, S& st) {
+  using node_t = typename C::node_t;
+  st.steps += 1;
+  if (c.failed()) {  // a corrupt object index was seen: abandon the operation
+    op_finish(st, 0);
+    return true;
+  }
+  if (st.steps > kMaxStepsPerOp) {
+    c.set_error(kErrStepLimit);
+    op_finish(st, 0);
+    return true;
+  }
+  const u32 q = c.key_slice(rq, st.layer);
+  const bool more = st.layer + 1 < rq.key_length;
+  switch (st.phase) {
+    case ph_descend: {
+      node_t& n = st.img;
+      if (!c.template read_node<Validate>(st.node, n)) return false;
+      const u32 h = c.header(n);
+      if (!hdr_border(h)) {
+        u32 child = c.find_child(n, q);
+        if (child == kInvalid) child = c.sibling(n);
+        if (child == kInvalid) {
+          c.set_error(kErrBadIndex);
+          op_finish(st, 0);
+          return true;
+        }
+        st.node = child;
+        return true;
+      }
+      if (q > c.high(n) && c.sibling(n) != kInvalid) {
+        st.node = c.sibling(n);
+        return true;
+      }
+      u32 ks = 0, val = 0;
+      const int pos = c.lookup(n, q, more, ks, val);
+      if (rq.kind == op_find) {
+        if (pos < 0) {
+          op_finish(st, 0);
+        } else if (ks == kLink) {
+          op_next_layer(st, val);
+        } else if (ks == kValue) {
+          c.write_inline_value(rq, val);
+          op_finish(st, 1);
+        } else {
+          st.a0 = ks;
+          st.node = val;
+          st.phase = ph_record;
+        }
+        return true;
+      }
+      if (pos >= 0 && ks == kLink) {
+        op_next_layer(st, val);
+        return true;
+      }
+      if (rq.kind == op_insert && pos >= 0 && !more) {  // key ends here and exists
+        op_finish(st, 0);
+        return true;
+      }
+      if ((rq.kind == op_erase || rq.kind == op_update) && pos < 0) {  // absent
+        op_finish(st, 0);
+        return true;
+      }
+      st.snap_hdr = h;
+      st.phase = ph_lock;
+      return true;
+    }
+
+    case ph_record: {
+      // records are immutable once published: no validation needed
+      u32 len_word;
+      if (!c.rec_load(st.node, len_word)) {
+        op_finish(st, 0);
+        return true;
+      }
+      if (st.a0 == kSuffix) {
+        const u32 tail = rq.key_length - st.layer - 1;
+        if (!c.rec_tail_equal(rq.key, st.layer + 1, tail, len_word)) {
+          op_finish(st, 0);
+          return true;
+        }
+      }
+      op_finish(st, c.rec_copy_value(rq, len_word));
+      return true;
+    }
+
+    case ph_lock: {
+      u32 old_hdr;
+      if (!c.try_lock(st.node, old_hdr)) return false;
+      node_t& n = st.img;
+      if (old_hdr != st.snap_hdr) c.read_node_locked(st.node, n);
+      // the in-memory header carries the lock bit while we hold the lock; keep the image
+      // identical so that write_node() never publishes an unlocked header
+      const u32 h = c.header(n) | kLockBit;
+      c.set_header(n, h);
+      if (!hdr_border(h)) {  // a layer root split in place since our read
+        c.unlock_same(st.node);
+        st.node = st.layer_root;
+        st.phase = ph_descend;
+        return true;
+      }
+      if (q > c.high(n) && c.sibling(n) != kInvalid) {  // split since our read: move right
+        c.unlock_same(st.node);
+        st.node = c.sibling(n);
+        st.snap_hdr = kInvalid;
+        return true;
+      }
+      u32 ks = 0, val = 0;
+      const int pos = c.lookup(n, q, more, ks, val);
+      if (pos >= 0 && ks == kLink) {
+        c.unlock_same(st.node);
+        op_next_layer(st, val);
+        return true;
+      }
+      if (pos < 0) {
+        if (rq.kind == op_erase || rq.kind == op_update) {
+          c.unlock_same(st.node);
+          op_finish(st, 0);
+          return true;
+        }
+        // insert a new entry
+        u32 nks, nval;
+        bool ok;
+        if (more) {
+          nks = kSuffix;
+          nval = c.rec_create(rq.key, st.layer + 1, rq.key_length - st.layer - 1, rq.value,
+                              rq.value_length);
+          ok = nval != kInvalid;
+        } else {
+          ok = make_value_entry(c, rq, nks, nval);
+        }
+        if (!ok) {
+          c.unlock_same(st.node);
+          op_finish(st, 0);
+          return true;
+        }
+        const u32 ipos = c.insert_position(n, q, more);
+        if (hdr_nkeys(h) < kBorderCap) {
+          c.border_insert(n, ipos, q, nks, nval);
+          c.write_node(st.node, n);
+          c.unlock_bump(st.node);
+          op_finish(st, 1);
+          return true;
+        }
+        // split the full border node
+        node_t left, right;
+        if (hdr_root(h)) {
+          const u32 li = c.alloc_slab();
+          const u32 ri = c.alloc_slab();
+          if (li == kInvalid || ri == kInvalid) {
+            c.unlock_same(st.node);
+            op_finish(st, 0);
+            return true;
+          }
+          const u32 sep = c.border_split(n, ipos, q, nks, nval, left, right);
+          c.set_header(left, hdr_make(c.nkeys_of(left), true, false, 0));
+          c.set_sibling(left, ri);
+          c.set_high(left, sep);
+          c.set_header(right, hdr_make(c.nkeys_of(right), true, false, 0));
+          c.set_sibling(right, kInvalid);
+          c.set_high(right, c.high(n));
+          c.write_node(li, left);
+          c.write_node(ri, right);
+          c.fence_release();
+          c.make_interior_root(n, sep, li, ri, 1, h);
+          c.write_node(st.node, n);
+          c.unlock_bump(st.node);
+          op_finish(st, 1);
+          return true;
+        }
+        const u32 ri = c.alloc_slab();
+        if (ri == kInvalid) {
+          c.unlock_same(st.node);
+          op_finish(st, 0);
+          return true;
+        }
+        const u32 sep = c.border_split(n, ipos, q, nks, nval, left, right);
+        c.set_header(right, hdr_make(c.nkeys_of(right), true, false, 0) | kLockBit);
+        c.set_sibling(right, c.sibling(n));
+        c.set_high(right, c.high(n));
+        c.set_header(left, hdr_with_nkeys(h, c.nkeys_of(left)));
+        c.set_sibling(left, ri);
+        c.set_high(left, sep);
+        c.write_node(ri, right);
+        c.write_node(st.node, left);
+        c.fence_release();
+        st.a0 = st.node;
+        st.a1 = ri;
+        st.a2 = sep;
+        st.a3 = 1;
+        st.result = 1;
+        st.node = st.layer_root;
+        st.snap_hdr = kInvalid;
+        st.phase = ph_parent_descend;
+        return true;
+      }
+      // the entry (q, more) exists: VALUE / LONGVAL (key ends here) or SUFFIX
+      if (ks == kSuffix) {
+        u32 len_word;
+        if (!c.rec_load(val, len_word)) {
+          c.unlock_same(st.node);
+          op_finish(st, 0);
+          return true;
+        }
+        const u32 tail = rq.key_length - st.layer - 1;
+        const bool equal = c.rec_tail_equal(rq.key, st.layer + 1, tail, len_word);
+        if (equal) {
+          if (rq.kind == op_insert) {
+            c.unlock_same(st.node);
+            op_finish(st, 0);
+            return true;
+          }
+          if (rq.kind == op_erase) {
+            c.border_remove(n, pos);
+            c.write_node(st.node, n);
+            c.unlock_bump(st.node);
+            c.retire(val);
+            op_finish(st, 1);
+            return true;
+          }
+          // update / upsert: same tail, new value
+          const u32 nrec = c.rec_create(rq.key, st.layer + 1, tail, rq.value, rq.value_length);
+          if (nrec == kInvalid) {
+            c.unlock_same(st.node);
+            op_finish(st, 0);
+            return true;
+          }
+          c.border_set(n, pos, kSuffix, nrec);
+          c.write_node(st.node, n);
+          c.unlock_bump(st.node);
+          c.retire(val);
+          op_finish(st, 1);
+          return true;
+        }
+        if (rq.kind == op_erase || rq.kind == op_update) {
+          c.unlock_same(st.node);
+          op_finish(st, 0);
+          return true;
+        }
+        // insert / upsert with a different tail: push the existing entry one layer down
+        const u32 xi = c.alloc_slab();
+        if (xi == kInvalid) {
+          c.unlock_same(st.node);
+          op_finish(st, 0);
+          return true;
+        }
+        u32 eks, eval;
+        u32 t0;
+        if (!c.rec_push_down(val, len_word, t0, eks, eval)) {
+          c.unlock_same(st.node);
+          op_finish(st, 0);
+          return true;
+        }
+        node_t x;
+        c.make_single_root(x, t0, eks, eval);
+        c.write_node(xi, x);
+        c.fence_release();
+        c.border_set(n, pos, kLink, xi);
+        c.write_node(st.node, n);
+        c.unlock_bump(st.node);
+        c.retire(val);
+        op_next_layer(st, xi);
+        return true;
+      }
+      // VALUE or LONGVAL
+      if (rq.kind == op_insert) {
+        c.unlock_same(st.node);
+        op_finish(st, 0);
+        return true;
+      }
+      if (rq.kind == op_erase) {
+        c.border_remove(n, pos);
+        c.write_node(st.node, n);
+        c.unlock_bump(st.node);
+        if (ks == kLongVal) c.retire(val);
+        op_finish(st, 1);
+        return true;
+      }
+      {
+        u32 nks, nval;
+        if (!make_value_entry(c, rq, nks, nval)) {
+          c.unlock_same(st.node);
+          op_finish(st, 0);
+          return true;
+        }
+        c.border_set(n, pos, nks, nval);
+        c.write_node(st.node, n);
+        c.unlock_bump(st.node);
+        if (ks == kLongVal) c.retire(val);
+        op_finish(st, 1);
+        return true;
+      }
+    }
+
+    case ph_parent_descend: {
+      node_t& n = st.img;
+      if (!c.template read_node<true>(st.node, n)) return false;
+      const u32 h = c.header(n);
+      const u32 lvl = hdr_level(h);
+      if (lvl < st.a3) {
+        c.set_error(kErrBadIndex);
+        op_finish(st, 0);
+        return true;
+      }
+      if (lvl == st.a3) {
+        st.snap_hdr = h;
+        st.phase = ph_parent_lock;
+        return true;
+      }
+      u32 child = c.find_child(n, st.a2);
+      if (child == kInvalid) child = c.sibling(n);
+      st.node = child;
+      return true;
+    }
+
+    case ph_parent_lock: {
+      u32 old_hdr;
+      if (!c.try_lock(st.node, old_hdr)) return false;
+      node_t& n = st.img;
+      if (old_hdr != st.snap_hdr) c.read_node_locked(st.node, n);
+      const u32 h = c.header(n) | kLockBit;
+      c.set_header(n, h);
+      if (hdr_border(h) || hdr_level(h) != st.a3) {  // root grew since our read
+        c.unlock_same(st.node);
+        st.node = st.layer_root;
+        st.phase = ph_parent_descend;
+        return true;
+      }
+      u32 j;
+      if (!c.find_child_slot(n, st.a0, j)) {
+        const u32 sib = c.sibling(n);
+        c.unlock_same(st.node);
+        if (sib != kInvalid) {
+          st.node = sib;
+          st.snap_hdr = kInvalid;
+        } else {
+          st.node = st.layer_root;
+          st.phase = ph_parent_descend;
+        }
+        return true;
+      }
+      if (hdr_nkeys(h) < kInteriorCap) {
+        c.interior_insert(n, j, st.a2, st.a0, st.a1);
+        c.write_node(st.node, n);
+        c.unlock_bump(st.node);
+        c.unlock_bump(st.a0);
+        c.unlock_bump(st.a1);
+        op_finish(st, st.result);
+        return true;
+      }
+      node_t left, right;
+      if (hdr_root(h)) {
+        const u32 li = c.alloc_slab();
+        const u32 ri = c.alloc_slab();
+        if (li == kInvalid || ri == kInvalid) {
+          // out of slabs: the separator cannot be placed; the split halves stay
+          // reachable through the sibling link, so the tree is still correct
+          c.unlock_same(st.node);
+          c.unlock_bump(st.a0);
+          c.unlock_bump(st.a1);
+          op_finish(st, st.result);
+          return true;
+        }
+        const u32 sep2 = c.interior_split(n, j, st.a2, st.a0, st.a1, left, right);
+        c.set_header(left, hdr_make(c.nkeys_of(left), false, false, st.a3));
+        c.set_sibling(left, ri);
+        c.set_header(right, hdr_make(c.nkeys_of(right), false, false, st.a3));
+        c.set_sibling(right, kInvalid);
+        c.write_node(li, left);
+        c.write_node(ri, right);
+        c.fence_release();
+        c.make_interior_root(n, sep2, li, ri, st.a3 + 1, h);
+        c.write_node(st.node, n);
+        c.unlock_bump(st.node);
+        c.unlock_bump(st.a0);
+        c.unlock_bump(st.a1);
+        op_finish(st, st.result);
+        return true;
+      }
+      const u32 ri = c.alloc_slab();
+      if (ri == kInvalid) {
+        c.unlock_same(st.node);
+        c.unlock_bump(st.a0);
+        c.unlock_bump(st.a1);
+        op_finish(st, st.result);
+        return true;
+      }
+      const u32 sep2 = c.interior_split(n, j, st.a2, st.a0, st.a1, left, right);
+      c.set_header(right, hdr_make(c.nkeys_of(right), false, false, st.a3) | kLockBit);
+      c.set_sibling(right, c.sibling(n));
+      c.set_header(left, hdr_with_nkeys(h, c.nkeys_of(left)));
+      c.set_sibling(left, ri);
+      c.write_node(ri, right);
+      c.write_node(st.node, left);
+      c.fence_release();
+      c.unlock_bump(st.a0);
+      c.unlock_bump(st.a1);
+      st.a0 = st.node;
+      st.a1 = ri;
+      st.a2 = sep2;
+      st.a3 = st.a3 + 1;
+      st.node = st.layer_root;
+      st.snap_hdr = kInvalid;
+      st.phase = ph_parent_descend;
+      return true;
+    }
+
+    default:
+      return true;
+  }
+}
+
+}  // namespace indexingpu_amd

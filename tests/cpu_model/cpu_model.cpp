@@ -1,0 +1,191 @@
+// CPU model of the Masstree batch engine (test infrastructure only).
+//
+// Runs the same per-tile state machine as the gfx950 kernels (tree_ops.hpp) with the
+// CPU tile policy, either
+//   * interleaved: many simulated tiles, one step at a time, tiles picked by a seeded
+//     random scheduler (reproducible interleavings at step granularity), or
+//   * threaded: one host thread per tile, real concurrency through GCC atomics.
+// The host walker then validates the structure and dumps the entries in key order.
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "cpu_policy.hpp"
+#include "indexingpu_amd/host_walk.hpp"
+
+using namespace indexingpu_amd;
+using cpu_model::cpu_tile;
+using cpu_model::shared_pool;
+
+namespace {
+struct model {
+  shared_pool pool;
+  u32 root;
+  explicit model(u32 n) : pool(n) {
+    cpu_tile t(pool);
+    root = t.alloc_slab();
+    cpu_model::node_img r;
+    std::memset(&r, 0, sizeof(r));
+    r.s[kHighSlot] = slot_make(kMaxSlice, 0);
+    r.s[kMetaSlot] = slot_make(hdr_make(0, true, true, 0), kInvalid);
+    t.write_node(root, r);
+  }
+};
+
+struct batch_args {
+  const uint8_t* kinds;
+  uint32_t default_kind;
+  const uint32_t* keys;
+  uint32_t maxK;
+  const uint32_t* klens;
+  const uint32_t* values;
+  uint32_t maxV;
+  const uint32_t* vlens;
+  uint32_t* out_values;
+  uint32_t* out_vlens;
+  uint8_t* results;
+};
+
+request make_request(const batch_args& a, uint64_t i, uint32_t* scratch_out) {
+  request rq;
+  rq.kind = a.kinds ? a.kinds[i] : a.default_kind;
+  rq.key_length = a.klens ? a.klens[i] : a.maxK;
+  rq.key = a.keys + i * (uint64_t)a.maxK;
+  rq.value = a.values ? a.values + i * (uint64_t)a.maxV : nullptr;
+  rq.value_length = a.vlens ? a.vlens[i] : a.maxV;
+  rq.out_value = a.out_values ? a.out_values + i * (uint64_t)a.maxV : scratch_out;
+  rq.max_value_length = a.out_values ? a.maxV : 0;
+  // a find in a mixed batch writes its value over the request's value slots
+  if (rq.kind == op_find && a.values && !a.out_values) {
+    rq.out_value = const_cast<uint32_t*>(rq.value);
+    rq.max_value_length = a.maxV;
+  }
+  return rq;
+}
+
+void store_result(const batch_args& a, const request& rq, uint64_t i, uint32_t result) {
+  if (rq.kind == op_find) {
+    if (a.out_vlens) a.out_vlens[i] = result;
+  } else if (a.results) {
+    a.results[i] = (uint8_t)result;
+  }
+}
+
+using state_t = op_state<cpu_model::node_img>;
+}  // namespace
+
+extern "C" {
+
+void* cm_create(uint32_t num_slabs) { return new model(num_slabs); }
+void cm_destroy(void* h) { delete static_cast<model*>(h); }
+uint32_t cm_error(void* h) { return static_cast<model*>(h)->pool.error.load(); }
+uint32_t cm_slabs_used(void* h) { return static_cast<model*>(h)->pool.next.load(); }
+uint64_t cm_retired(void* h) { return static_cast<model*>(h)->pool.retired.size(); }
+
+// Runs one batch.  nthreads == 0: interleaved simulation with ntiles tiles and the
+// given scheduler seed; nthreads > 0: that many host threads (static partition).
+void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32_t* keys,
+              uint32_t maxK, const uint32_t* klens, const uint32_t* values, uint32_t maxV,
+              const uint32_t* vlens, uint32_t* out_values, uint32_t* out_vlens,
+              uint8_t* results, uint64_t n, uint32_t ntiles, uint32_t nthreads,
+              uint64_t seed) {
+  model& M = *static_cast<model*>(h);
+  batch_args a{kinds, default_kind, keys, maxK, klens, values, maxV, vlens,
+               out_values, out_vlens, results};
+  if (nthreads == 0) {
+    std::mt19937_64 rng(seed);
+    std::vector<state_t> st(ntiles);
+    std::vector<uint64_t> cur(ntiles);
+    std::vector<request> rq(ntiles);
+    std::vector<cpu_tile> ctx;
+    ctx.reserve(ntiles);
+    std::vector<uint32_t> scratch(ntiles, 0);
+    std::vector<uint32_t> active;
+    for (uint32_t t = 0; t < ntiles; t++) {
+      ctx.emplace_back(M.pool);
+      cur[t] = t;
+      if (t < n) {
+        rq[t] = make_request(a, t, &scratch[t]);
+        op_begin(ctx[t], st[t], M.root);
+        if (rq[t].key_length == 0) op_finish(st[t], 0);
+        active.push_back(t);
+      }
+    }
+    while (!active.empty()) {
+      size_t pick = rng() % active.size();
+      uint32_t t = active[pick];
+      if (st[t].phase != ph_done) op_step<true>(ctx[t], rq[t], st[t]);
+      if (st[t].phase == ph_done) {
+        store_result(a, rq[t], cur[t], st[t].result);
+        cur[t] += ntiles;
+        if (cur[t] < n) {
+          rq[t] = make_request(a, cur[t], &scratch[t]);
+          op_begin(ctx[t], st[t], M.root);
+          if (rq[t].key_length == 0) op_finish(st[t], 0);
+        } else {
+          active[pick] = active.back();
+          active.pop_back();
+        }
+      }
+    }
+    return;
+  }
+  std::vector<std::thread> th;
+  for (uint32_t t = 0; t < nthreads; t++) {
+    th.emplace_back([&, t]() {
+      cpu_tile c(M.pool);
+      state_t s;
+      uint32_t scratch = 0;
+      for (uint64_t i = t; i < n; i += nthreads) {
+        request r = make_request(a, i, &scratch);
+        op_begin(c, s, M.root);
+        if (r.key_length == 0) op_finish(s, 0);
+        while (s.phase != ph_done) {
+          if (!op_step<true>(c, r, s)) std::this_thread::yield();
+        }
+        store_result(a, r, i, s.result);
+      }
+    });
+  }
+  for (auto& x : th) x.join();
+}
+
+// Validates the structure and dumps entries in key order (same buffer protocol as
+// orc_dump).  Returns the entry count, or ~0 on a structural error (message in errbuf).
+uint64_t cm_dump(void* h, uint32_t* key_offsets, uint32_t* key_data, uint32_t* value_offsets,
+                 uint32_t* value_data, uint64_t* key_slices, uint64_t* value_slices,
+                 char* errbuf, uint32_t errlen, uint64_t* nodes) {
+  model& M = *static_cast<model*>(h);
+  uint64_t ks = 0, vs = 0, i = 0;
+  host_walker w(
+      [&](uint32_t idx) -> const uint64_t* {
+        return idx < M.pool.num_slabs ? M.pool.slab(idx) : nullptr;
+      },
+      [&](const std::vector<uint32_t>& k, const std::vector<uint32_t>& v) {
+        if (key_offsets) key_offsets[i] = (uint32_t)ks;
+        if (value_offsets) value_offsets[i] = (uint32_t)vs;
+        if (key_data) std::memcpy(key_data + ks, k.data(), k.size() * 4);
+        if (value_data) std::memcpy(value_data + vs, v.data(), v.size() * 4);
+        ks += k.size();
+        vs += v.size();
+        i++;
+      });
+  walk_stats s = w.run(M.root);
+  if (key_offsets) key_offsets[i] = (uint32_t)ks;
+  if (value_offsets) value_offsets[i] = (uint32_t)vs;
+  if (key_slices) *key_slices = ks;
+  if (value_slices) *value_slices = vs;
+  if (nodes) *nodes = s.nodes;
+  if (!s.error.empty()) {
+    if (errbuf && errlen) {
+      std::strncpy(errbuf, s.error.c_str(), errlen - 1);
+      errbuf[errlen - 1] = 0;
+    }
+    return ~0ull;
+  }
+  return i;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+"""ctypes wrapper of libcpu_model.so (test infrastructure only)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libcpu_model.so")
+_lib = None
+u32p = ctypes.POINTER(ctypes.c_uint32)
+u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        subprocess.check_call(["make", "-s", "-C", _HERE])
+        lib = ctypes.CDLL(_LIB)
+        vp, u32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+        lib.cm_create.argtypes = [u32]
+        lib.cm_create.restype = vp
+        lib.cm_destroy.argtypes = [vp]
+        lib.cm_error.argtypes = [vp]
+        lib.cm_error.restype = u32
+        lib.cm_slabs_used.argtypes = [vp]
+        lib.cm_slabs_used.restype = u32
+        lib.cm_retired.argtypes = [vp]
+        lib.cm_retired.restype = u64
+        lib.cm_batch.argtypes = [vp, u8p, u32, u32p, u32, u32p, u32p, u32, u32p, u32p, u32p,
+                                 u8p, u64, u32, u32, u64]
+        lib.cm_dump.argtypes = [vp, u32p, u32p, u32p, u32p, ctypes.POINTER(u64),
+                                ctypes.POINTER(u64), ctypes.c_char_p, u32, ctypes.POINTER(u64)]
+        lib.cm_dump.restype = u64
+        _lib = lib
+    return _lib
+
+
+def _p32(a):
+    return None if a is None else a.ctypes.data_as(u32p)
+
+
+def _p8(a):
+    return None if a is None else a.ctypes.data_as(u8p)
+
+
+class CpuModel:
+    def __init__(self, num_slabs=1 << 20):
+        self.lib = _load()
+        self.h = self.lib.cm_create(num_slabs)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.cm_destroy(self.h)
+            self.h = None
+
+    @property
+    def error(self):
+        return int(self.lib.cm_error(self.h))
+
+    def batch(self, kind, keys, maxK, klens, values=None, maxV=1, vlens=None, n=None,
+              kinds=None, want_find_out=False, ntiles=8, nthreads=0, seed=0, mixed=False):
+        n = n if n is not None else (len(klens) if klens is not None else keys.size // maxK)
+        out_values = out_vlens = None
+        if want_find_out:
+            out_values = np.zeros(n * maxV, np.uint32)
+            out_vlens = np.zeros(n, np.uint32)
+        results = np.zeros(n, np.uint8)
+        if mixed:
+            values = values.copy()
+            out_vlens = (vlens.copy() if vlens is not None
+                         else np.full(n, maxV, np.uint32))
+            vlens = out_vlens
+        self.lib.cm_batch(self.h, _p8(kinds), kind, _p32(keys), maxK, _p32(klens),
+                          _p32(values), maxV, _p32(vlens), _p32(out_values), _p32(out_vlens),
+                          _p8(results), n, ntiles, nthreads, seed)
+        if mixed:
+            return values, out_vlens, results
+        return out_values, out_vlens, results
+
+    def dump(self):
+        ks, vs, nodes = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        err = ctypes.create_string_buffer(256)
+        n = self.lib.cm_dump(self.h, None, None, None, None, ctypes.byref(ks), ctypes.byref(vs),
+                             err, 256, ctypes.byref(nodes))
+        if n == 2**64 - 1:
+            raise AssertionError("structure invalid: " + err.value.decode())
+        ko = np.zeros(n + 1, np.uint32)
+        kd = np.zeros(max(ks.value, 1), np.uint32)
+        vo = np.zeros(n + 1, np.uint32)
+        vd = np.zeros(max(vs.value, 1), np.uint32)
+        self.lib.cm_dump(self.h, _p32(ko), _p32(kd), _p32(vo), _p32(vd), None, None, err, 256,
+                         None)
+        return ko, kd[:ks.value], vo, vd[:vs.value]
+
+    def items(self):
+        ko, kd, vo, vd = self.dump()
+        return [(tuple(kd[ko[i]:ko[i + 1]].tolist()), tuple(vd[vo[i]:vo[i + 1]].tolist()))
+                for i in range(len(ko) - 1)]

@@ -1,0 +1,80 @@
+"""The batch engine's state machine (include/indexingpu_amd/tree_ops.hpp) run on the host
+with the CPU tile policy, against the oracle: reference scenarios with seeded random
+interleavings of many tiles, and real concurrency with 8 host threads.  Parity is exact:
+find outputs, per-op results and the full ordered dump."""
+import numpy as np
+import pytest
+
+import refinputs as R
+import scenarios as S
+from indexingpu_amd import workloads as W
+
+PLAIN = [S.sc_find_exist, S.sc_find_notexist, S.sc_update, S.sc_erase_all, S.sc_erase_none,
+         S.sc_erase_all_insert_all]
+
+
+def _same(a, b, skip=()):
+    assert a.keys() == b.keys()
+    for k in a:
+        if k in skip:
+            continue
+        if isinstance(a[k], list):
+            assert a[k] == b[k], k
+        else:
+            assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("nthreads", [0, 8])
+@pytest.mark.parametrize("c", range(4))
+def test_reference_scenarios(c, nthreads):
+    for sc in PLAIN:
+        inp = R.TestingInput(c)
+        _same(sc(S.CpuModelBackend(nthreads=nthreads), inp), sc(S.OracleBackend(), inp))
+    inp = R.TestingInput(c)
+    mix = R.ConcurrentMix(inp)
+    got = S.sc_concurrent_mix(S.CpuModelBackend(nthreads=nthreads), inp, mix)
+    inp = R.TestingInput(c)
+    mix = R.ConcurrentMix(inp)
+    exp = S.sc_concurrent_mix(S.OracleBackend(), inp, mix)
+    # finds of keys written in the same batch are order-dependent (reference :657-676)
+    ks = mix.num_keyset
+    det = (mix.dst_of < 4 * ks) | (mix.dst_of >= 7 * ks)
+    assert np.array_equal(got["mix_results"][det], exp["mix_results"][det])
+    assert np.array_equal(got["mix_lens"][det], exp["mix_lens"][det])
+    _same({k: got[k] for k in ("find_values", "find_lens", "items")},
+          {k: exp[k] for k in ("find_values", "find_lens", "items")})
+
+
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_duplicates_in_batch(nthreads):
+    for c in range(4):
+        for sc in (S.sc_insert_twice_erase_all, S.sc_erase_all_twice):
+            got = sc(S.CpuModelBackend(nthreads=nthreads), R.TestingInput(c, duplicate=True))
+            exp = sc(S.OracleBackend(), R.TestingInput(c, duplicate=True))
+            # which copy of a duplicated key wins is order-dependent; totals are not
+            for k in ("insert_dup", "erase_dup"):
+                if k in got:
+                    assert got[k].sum() == exp[k].sum()
+            _same({k: got[k] for k in ("find_values", "find_lens", "items")},
+                  {k: exp[k] for k in ("find_values", "find_lens", "items")})
+
+
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_random_mixed_batches(nthreads):
+    wl = W.MixedWorkload(20000, key_len=3, seed=11)
+    # shared first slices force layers and suffix push-downs
+    wl.keys[:, 0] %= 64
+    wl.hashes = W.key_hash64(wl.keys, wl.lens)
+    b = S.CpuModelBackend(nthreads=nthreads)
+    o = S.OracleBackend()
+    mk, mv = 3, 2
+    assert np.array_equal(b.insert(wl.keys, wl.lens, wl.values, None, mk, mv),
+                          o.insert(wl.keys, wl.lens, wl.values, None, mk, mv))
+    for _ in range(3):
+        bt = wl.batch(4000)
+        vl = np.full(len(bt["types"]), mv, np.uint32)
+        got = b.mixed(bt["types"], bt["keys"], bt["lens"], bt["values"], vl, mk, mv)
+        exp = o.mixed(bt["types"], bt["keys"], bt["lens"], bt["values"], vl, mk, mv)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e)
+    assert b.items() == o.items()

@@ -1,0 +1,152 @@
+"""GPU parity on the benchmark workload shapes, at sizes the oracle replays in seconds,
+plus edge cases.  Every comparison is bit-exact against the oracle on the same inputs."""
+import numpy as np
+import pytest
+
+import scenarios as S
+from indexingpu_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq(a, b):
+    assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+@pytest.mark.parametrize("tile", [16, 64])
+def test_plain_batch_varlen(tile):
+    """Plain batch path shape: var-len keys of 2..8 slices, 2-slice values, one find
+    batch with 50% hits, one erase batch, then a find of everything."""
+    d = W.plain_batch_workload(200_000, seed=5)
+    mk, mv = d["max_key_length"], d["max_value_length"]
+    g, o = S.GpuBackend(tile, pool_bytes=256 << 20), S.OracleBackend()
+    _eq(g.insert(d["keys"], d["lens"], d["values"], None, mk, mv),
+        o.insert(d["keys"], d["lens"], d["values"], None, mk, mv))
+    for a, b in zip(g.find(d["find_keys"], d["find_lens"], mk, mv),
+                    o.find(d["find_keys"], d["find_lens"], mk, mv)):
+        _eq(a, b)
+    _eq(g.erase(d["erase_keys"], d["erase_lens"], mk), o.erase(d["erase_keys"], d["erase_lens"], mk))
+    for a, b in zip(g.find(d["keys"], d["lens"], mk, mv, concurrent=True),
+                    o.find(d["keys"], d["lens"], mk, mv)):
+        _eq(a, b)
+    assert g.items() == o.items()
+    ok, *_, msg = g.validate()
+    assert ok, msg
+
+
+def test_cpu_path_shape_fixed_length():
+    """CPU-runnable case shape: fixed 2-slice keys (key_lengths = null), 1-slice values."""
+    d = W.cpu_path_workload(100_000, seed=7)
+    g, o = S.GpuBackend(16, pool_bytes=128 << 20), S.OracleBackend()
+    n = len(d["keys"])
+    import torch
+    keys = g._d(d["keys"])
+    vals = g._d(d["values"])
+    g.t.insert(keys, 2, None, vals, 1, None, n)
+    o.o.insert(d["keys"].reshape(-1).copy(), 2, None, d["values"].reshape(-1).copy(), 1, None, n)
+    fv = g._zeros(n)
+    fl = g._zeros(n)
+    g.t.find(g._d(d["find_keys"]), 2, None, fv, 1, fl, n)
+    ev, el = o.o.find(d["find_keys"].reshape(-1).copy(), 2, None, 1, n)
+    _eq(g._h(fv)[:n], ev)
+    _eq(g._h(fl)[:n], el)
+    g.t.check_device_error()
+    assert int((el > 0).sum()) == n // 2
+
+
+@pytest.mark.parametrize("tile", [16, 32])
+def test_mixed_batches_and_scans(tile):
+    """Mixed shape: Zipf(0.99) finds / fresh inserts / erases of unread keys, then range
+    scans with random lower bounds and per-query limits in [1, 100]."""
+    wl = W.MixedWorkload(200_000, key_len=4, seed=13)
+    mk, mv = 4, 2
+    g, o = S.GpuBackend(tile, pool_bytes=256 << 20), S.OracleBackend()
+    _eq(g.insert(wl.keys, wl.lens, wl.values, None, mk, mv),
+        o.insert(wl.keys, wl.lens, wl.values, None, mk, mv))
+    for _ in range(4):
+        bt = wl.batch(50_000)
+        vl = np.full(len(bt["types"]), mv, np.uint32)
+        for a, b in zip(g.mixed(bt["types"], bt["keys"], bt["lens"], bt["values"], vl, mk, mv),
+                        o.mixed(bt["types"], bt["keys"], bt["lens"], bt["values"], vl, mk, mv)):
+            _eq(a, b)
+    sc = wl.scans(4096, max_scan=100)
+    nq = len(sc["limits"])
+    lens = np.full(nq, mk, np.uint32)
+    for a, b in zip(g.scan(sc["lower"], lens, mk, 100, maxV=mv, limits=sc["limits"]),
+                    o.scan(sc["lower"], lens, mk, 100, maxV=mv, limits=sc["limits"])):
+        _eq(a, b)
+    assert g.items() == o.items()
+
+
+def test_layers_push_down_and_long_keys():
+    """Keys sharing long prefixes (trie layers, suffix push-downs) and keys longer than
+    a tile (global key reads, multi-slab records), with scans across layers."""
+    rng = np.random.default_rng(21)
+    n, mk, mv = 20_000, 80, 40
+    lens = rng.integers(1, mk + 1, size=n).astype(np.uint32)
+    keys = rng.integers(0, 4, size=(n, mk)).astype(np.uint32)
+    keys[:, :40] = (keys[:, :40] % 2)  # long shared prefixes
+    keys[np.arange(mk)[None, :] >= lens[:, None]] = 0
+    h = W.key_hash64(keys, lens)
+    _, first = np.unique(h, return_index=True)
+    first = np.sort(first)
+    keys, lens = np.ascontiguousarray(keys[first]), lens[first]
+    n = len(lens)
+    vlens = rng.integers(0, mv + 1, size=n).astype(np.uint32)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    for tile in (16, 64):
+        g, o = S.GpuBackend(tile, pool_bytes=512 << 20), S.OracleBackend()
+        half = n // 2
+        _eq(g.insert(keys[:half], lens[:half], vals[:half], vlens[:half], mk, mv),
+            o.insert(keys[:half], lens[:half], vals[:half], vlens[:half], mk, mv))
+        _eq(g.insert(keys, lens, vals, vlens, mk, mv), o.insert(keys, lens, vals, vlens, mk, mv))
+        for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+            _eq(a, b)
+        q = rng.choice(n, 500, replace=False)
+        u = rng.choice(n, 500, replace=False)
+        for a, b in zip(g.scan(keys[q], lens[q], mk, 30, keys[u], lens[u], maxV=mv),
+                        o.scan(keys[q], lens[q], mk, 30, keys[u], lens[u], maxV=mv)):
+            _eq(a, b)
+        e = rng.choice(n, n // 3, replace=False)
+        _eq(g.erase(keys[e], lens[e], mk), o.erase(keys[e], lens[e], mk))
+        assert g.items() == o.items()
+        ok, *_, msg = g.validate()
+        assert ok, msg
+
+
+def test_upsert_update_and_zero_length_values():
+    """insert<update_if_exists=true>, update, and values of length 0 (stored, read back
+    as length 0, but the key still exists for insert/update/erase)."""
+    rng = np.random.default_rng(3)
+    keys, lens = W.unique_random_keys(rng, 30_000, 1, 6)
+    mk, mv = 6, 3
+    vl = rng.integers(0, mv + 1, size=len(lens)).astype(np.uint32)
+    vals = rng.integers(0, 1 << 32, size=(len(lens), mv), dtype=np.uint64).astype(np.uint32)
+    g, o = S.GpuBackend(16, pool_bytes=128 << 20), S.OracleBackend()
+    _eq(g.insert(keys, lens, vals, vl, mk, mv), o.insert(keys, lens, vals, vl, mk, mv))
+    vl2 = rng.integers(0, mv + 1, size=len(lens)).astype(np.uint32)
+    vals2 = vals ^ np.uint32(0x5A5A5A5A)
+    half = len(lens) // 2
+    _eq(g.insert(keys[:half], lens[:half], vals2[:half], vl2[:half], mk, mv, upsert=True),
+        o.insert(keys[:half], lens[:half], vals2[:half], vl2[:half], mk, mv, upsert=True))
+    _eq(g.update(keys[half:], lens[half:], vals2[half:], vl2[half:], mk, mv),
+        o.update(keys[half:], lens[half:], vals2[half:], vl2[half:], mk, mv))
+    _eq(g.insert(keys, lens, vals, vl, mk, mv), o.insert(keys, lens, vals, vl, mk, mv))
+    for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        _eq(a, b)
+    assert g.items() == o.items()
+
+
+def test_empty_batches_and_tiny_pool_error():
+    g = S.GpuBackend(16, pool_bytes=1 << 16)  # 512 slabs
+    import torch
+    e = g._zeros(1)
+    g.t.find(e, 1, None, e, 1, e, 0)
+    g.t.insert(e, 1, None, e, 1, None, 0)
+    g.t.check_device_error()
+    rng = np.random.default_rng(0)
+    keys, lens = W.unique_random_keys(rng, 20_000, 3, 3)
+    vals = np.zeros((len(lens), 2), np.uint32)
+    r = g._zeros(len(lens), np.uint8)
+    g.t.insert(g._d(keys), 3, g._d(lens), g._d(vals), 2, None, len(lens), results=r)
+    assert g.t.device_error() == 1  # pool exhausted is reported, not a hang
