@@ -120,6 +120,10 @@ int64_t gmt_dump(gmt_index* index, uint32_t* key_offsets, uint32_t* key_data,
 uint64_t gmt_slabs_allocated(gmt_index* index);
 uint64_t gmt_pool_slabs(gmt_index* index);
 
+/* Lookups each tile keeps in flight in read-only find batches (1, 2, 4 or 8; other
+ * values select 4).  A tuning knob with no counterpart in the reference. */
+int gmt_set_find_depth(gmt_index* index, uint32_t depth);
+
 /* Tile width the index was created with. */
 uint32_t gmt_tile_size(gmt_index* index);
 

@@ -51,7 +51,7 @@ struct device_allocator_context {
   __device__ device_allocator_context(const slab_pool_view& v, const Tile&) : view(v) {}
   __device__ ~device_allocator_context() {
     if (cache.bits != 0 && (__lane_id() & 15u) == 0)
-      dev::atomic_and_relaxed(&view.bitmap[cache.word], ~cache.bits);
+      dev::atomic_and(&view.bitmap[cache.word], ~cache.bits);
   }
 };
 
@@ -148,8 +148,21 @@ struct gpu_masstree {
     a.max_value_length = max_value_length;
     a.out_values = values;
     a.out_value_lengths = value_lengths;
-    launch_batch<concurrent>(a, stream, false);
+    if (concurrent) {
+      launch_batch<true>(a, stream, false);
+    } else {
+      switch (find_depth_) {
+        case 1: launch_find<1>(a, stream); break;
+        case 2: launch_find<2>(a, stream); break;
+        case 8: launch_find<8>(a, stream); break;
+        default: launch_find<4>(a, stream); break;
+      }
+    }
   }
+
+  // Lookups kept in flight per tile by the read-only find kernel (1, 2, 4 or 8).
+  void set_find_depth(uint32_t k) { find_depth_ = (k == 1 || k == 2 || k == 8) ? k : 4; }
+  uint32_t find_depth() const { return find_depth_; }
 
   template <bool update_if_exists = false, bool enable_suffix = true, bool use_shmem_key = false>
   void insert(const key_slice_type* keys, const size_type max_key_length,
@@ -256,7 +269,8 @@ struct gpu_masstree {
   __device__ size_type cooperative_find(K& key, size_type key_length, V& value,
                                         size_type max_value_length, const tile_type& tile,
                                         device_allocator_context_type& allocator) {
-    indexingpu_amd::wave_tile<tile_size> c(allocator.view, indexingpu_amd::retire_view{}, err_);
+    indexingpu_amd::wave_tile<tile_size, concurrent> c(allocator.view, indexingpu_amd::retire_view{},
+                                                      err_);
     indexingpu_amd::request rq{indexingpu_amd::op_find, key_length, as_ptr(key), nullptr, 0,
                                as_mut_ptr(value), max_value_length};
     return run_op<concurrent>(c, rq);
@@ -268,7 +282,7 @@ struct gpu_masstree {
                                      const size_type value_length, const tile_type& tile,
                                      device_allocator_context_type& allocator,
                                      device_reclaimer_context_type& reclaimer) {
-    indexingpu_amd::wave_tile<tile_size> c(allocator.view, reclaimer.view, err_);
+    indexingpu_amd::wave_tile<tile_size, true> c(allocator.view, reclaimer.view, err_);
     c.ac = allocator.cache;
     c.rc = reclaimer.cache;
     indexingpu_amd::request rq{update_if_exists ? indexingpu_amd::op_upsert : indexingpu_amd::op_insert,
@@ -284,7 +298,7 @@ struct gpu_masstree {
                                      const size_type value_length, const tile_type& tile,
                                      device_allocator_context_type& allocator,
                                      device_reclaimer_context_type& reclaimer) {
-    indexingpu_amd::wave_tile<tile_size> c(allocator.view, reclaimer.view, err_);
+    indexingpu_amd::wave_tile<tile_size, true> c(allocator.view, reclaimer.view, err_);
     c.ac = allocator.cache;
     c.rc = reclaimer.cache;
     indexingpu_amd::request rq{indexingpu_amd::op_update, key_length, as_ptr(key), as_ptr(value),
@@ -300,7 +314,7 @@ struct gpu_masstree {
   __device__ bool cooperative_erase(K& key, const size_type key_length, const tile_type& tile,
                                     device_allocator_context_type& allocator,
                                     device_reclaimer_context_type& reclaimer) {
-    indexingpu_amd::wave_tile<tile_size> c(allocator.view, reclaimer.view, err_);
+    indexingpu_amd::wave_tile<tile_size, true> c(allocator.view, reclaimer.view, err_);
     c.ac = allocator.cache;
     c.rc = reclaimer.cache;
     indexingpu_amd::request rq{indexingpu_amd::op_erase, key_length, as_ptr(key), nullptr, 0,
@@ -389,8 +403,8 @@ struct gpu_masstree {
     return static_cast<uint32_t*>(v);
   }
 
-  template <bool Validate>
-  __device__ size_type run_op(indexingpu_amd::wave_tile<tile_size>& c,
+  template <bool Validate, class Tile>
+  __device__ size_type run_op(Tile& c,
                               const indexingpu_amd::request& rq) {
     if (rq.key_length == 0) {
       c.set_error(indexingpu_amd::kErrBadLength);
@@ -399,8 +413,15 @@ struct gpu_masstree {
     c.load_key_cache(rq);
     indexingpu_amd::op_state<indexingpu_amd::gpu_node> st;
     indexingpu_amd::op_begin(c, st, root_index_);
+    uint32_t backoff = 0;
     while (st.phase != indexingpu_amd::ph_done) {
-      if (!indexingpu_amd::op_step<Validate>(c, rq, st)) __builtin_amdgcn_s_sleep(1);
+      if (indexingpu_amd::op_step<Validate>(c, rq, st) || indexingpu_amd::op_holds_locks(st)) {
+        backoff = 0;
+        continue;
+      }
+      if (backoff < indexingpu_amd::kMaxBackoffShift) backoff += 1;
+      for (uint32_t k = 0; k < (1u << backoff); k++) __builtin_amdgcn_s_sleep(1);
+      st.steps += 1u << backoff;
     }
     return st.result;
   }
@@ -431,6 +452,17 @@ struct gpu_masstree {
     if (mutating) drain(stream);
   }
 
+  template <uint32_t K>
+  void launch_find(const indexingpu_amd::batch_args& a, hipStream_t stream) {
+    if (a.n == 0) return;
+    auto kernel = indexingpu_amd::find_kernel<tile_size, K>;
+    const uint32_t per_block = indexingpu_amd::kBlockThreads / tile_size * K;
+    uint64_t want = (a.n + per_block - 1) / per_block;
+    uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(kernel));
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
+                       view(), a);
+  }
+
   void ensure_scan_stack(uint64_t words) {
     if (words <= scan_stack_words_) return;
     IGPU_HIP_TRY(hipDeviceSynchronize());
@@ -453,6 +485,7 @@ struct gpu_masstree {
   }
 
   bool owner_ = true;
+  uint32_t find_depth_ = 4;
   uint32_t* scan_stack_ = nullptr;
   uint64_t scan_stack_words_ = 0;
 };

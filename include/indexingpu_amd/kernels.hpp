@@ -19,6 +19,7 @@
 namespace indexingpu_amd {
 
 constexpr u32 kBlockThreads = 256;
+constexpr u32 kMaxBackoffShift = 8;
 
 struct tree_view {
   slab_pool_view pool;
@@ -60,13 +61,13 @@ __device__ __forceinline__ void load_request(const batch_args& a, u64 i, request
 // After a corruption or livelock guard fired, the remaining requests of the launch are
 // skipped so that the kernel drains quickly.
 __device__ __forceinline__ bool abort_pending(u32* err) {
-  const u32 e = dev::load_relaxed(err);
+  const u32 e = dev::coherent_load(err);
   return e == kErrStepLimit || e == kErrBadIndex;
 }
 
 template <u32 T, bool Validate>
 __global__ void __launch_bounds__(kBlockThreads) batch_kernel(tree_view tv, batch_args a) {
-  wave_tile<T> c(tv.pool, tv.retire, tv.err);
+  wave_tile<T, Validate> c(tv.pool, tv.retire, tv.err);
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
   u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
@@ -89,10 +90,30 @@ __global__ void __launch_bounds__(kBlockThreads) batch_kernel(tree_view tv, batc
     }
   };
   if (active) start();
+  // A tile whose step could not progress (lock busy, torn read) sits out a randomized,
+  // exponentially growing number of iterations before retrying, so hot nodes are not
+  // hammered by thousands of tiles; waiting iterations count toward the step budget.
+  u32 backoff = 0, wait = 0;
   while (active) {
     bool progressed = true;
-    if (st.phase != ph_done) progressed = op_step<Validate>(c, rq, st);
+    if (wait > 0) {
+      wait -= 1;
+      st.steps += 1;
+      progressed = false;
+    } else if (st.phase != ph_done) {
+      progressed = op_step<Validate>(c, rq, st);
+      if (progressed || op_holds_locks(st)) {
+        backoff = 0;
+      } else {
+        if (backoff < kMaxBackoffShift) backoff += 1;
+        const u32 span = 1u << backoff;
+        const u32 jitter = (u32)(i * 0x9E3779B1u) ^ (st.steps * 0x85EBCA6Bu);
+        wait = span - 1 + ((jitter >> 7) & (span - 1));
+      }
+    }
     if (st.phase == ph_done) {
+      backoff = 0;
+      wait = 0;
       if (c.lane == 0) {
         if (rq.kind == op_find) {
           if (a.out_value_lengths) a.out_value_lengths[i] = st.result;
@@ -104,8 +125,8 @@ __global__ void __launch_bounds__(kBlockThreads) batch_kernel(tree_view tv, batc
       active = i < a.n;
       if (active) start();
     }
-    // every tile of this wave is waiting on a lock: back off briefly
-    if (__ballot(!progressed) == __ballot(true)) __builtin_amdgcn_s_sleep(2);
+    // every tile of this wave is waiting: let the other waves of the SIMD run
+    if (__ballot(!progressed) == __ballot(true)) __builtin_amdgcn_s_sleep(1);
   }
   c.flush();
 }
@@ -155,8 +176,8 @@ struct scan_query {
 };
 
 // number of entries e with (e.slice, e.more) <= (s, m)
-template <u32 T>
-__device__ __forceinline__ u32 entries_le(const wave_tile<T>& c, const gpu_node& n, u32 s, bool m) {
+template <u32 T, bool C>
+__device__ __forceinline__ u32 entries_le(const wave_tile<T, C>& c, const gpu_node& n, u32 s, bool m) {
   const u32 nk = c.nkeys_of(n);
   const u32 myks = ks_get(c.ksbits(n), c.slot);
   return (u32)__builtin_popcount(
@@ -164,8 +185,8 @@ __device__ __forceinline__ u32 entries_le(const wave_tile<T>& c, const gpu_node&
 }
 
 // compares record tail (record head loaded in c) with key[off, off+len): -1, 0, +1
-template <u32 T>
-__device__ __forceinline__ int rec_tail_cmp(wave_tile<T>& c, const u32* key, u32 off, u32 len,
+template <u32 T, bool C>
+__device__ __forceinline__ int rec_tail_cmp(wave_tile<T, C>& c, const u32* key, u32 off, u32 len,
                                             u32 len_word) {
   const u32 tl = rec_tail_len(len_word);
   const u32 common = tl < len ? tl : len;
@@ -198,8 +219,8 @@ __device__ __forceinline__ int rec_tail_cmp(wave_tile<T>& c, const u32* key, u32
 }
 
 // writes the output key for an entry: stack prefix (layers 0..l-1), slice, record tail
-template <u32 T>
-__device__ __forceinline__ void scan_write_key_prefix(wave_tile<T>& c, const scan_query<T>& q,
+template <u32 T, bool C>
+__device__ __forceinline__ void scan_write_key_prefix(wave_tile<T, C>& c, const scan_query<T>& q,
                                                       u32 out, u32 l, u32 slice) {
   if (!q.keys) return;
   u32* dst = q.keys + (u64)out * q.maxK;
@@ -208,8 +229,8 @@ __device__ __forceinline__ void scan_write_key_prefix(wave_tile<T>& c, const sca
   }
 }
 
-template <u32 T, bool Validate>
-__device__ __forceinline__ void scan_read(wave_tile<T>& c, u32 idx, gpu_node& n) {
+template <u32 T, bool Validate, bool C>
+__device__ __forceinline__ void scan_read(wave_tile<T, C>& c, u32 idx, gpu_node& n) {
   for (u32 spin = 0; spin < (1u << 20); spin++) {
     if (c.template read_node<Validate>(idx, n)) return;
     if (c.failed()) return;
@@ -220,8 +241,8 @@ __device__ __forceinline__ void scan_read(wave_tile<T>& c, u32 idx, gpu_node& n)
 }
 
 // Returns the number of results.
-template <u32 T, bool Validate>
-__device__ u32 tile_scan(wave_tile<T>& c, u32 root, const scan_query<T>& q) {
+template <u32 T, bool Validate, bool C>
+__device__ u32 tile_scan(wave_tile<T, C>& c, u32 root, const scan_query<T>& q) {
   if (q.limit == 0) return 0;
   u32 count = 0;
   u32 l = 0;
@@ -395,7 +416,7 @@ __device__ u32 tile_scan(wave_tile<T>& c, u32 root, const scan_query<T>& q) {
 
 template <u32 T, bool Validate>
 __global__ void __launch_bounds__(kBlockThreads) scan_kernel(tree_view tv, scan_args a) {
-  wave_tile<T> c(tv.pool, tv.retire, tv.err);
+  wave_tile<T, Validate> c(tv.pool, tv.retire, tv.err);
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
   const u64 gtile = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
@@ -454,7 +475,7 @@ __global__ void reset_kernel(retire_view rv) {
 
 __global__ void init_root_kernel(slab_pool_view pool, u32* err, u32* out_root) {
   if (threadIdx.x >= 16) return;
-  wave_tile<16> c(pool, retire_view{}, err);
+  wave_tile<16, true> c(pool, retire_view{}, err);
   const u32 r = c.alloc_slab();
   if (r != kInvalid) {
     gpu_node n;
@@ -467,5 +488,127 @@ __global__ void init_root_kernel(slab_pool_view pool, u32* err, u32* out_root) {
   if (c.lane == 0) *out_root = r;
 }
 
-// Copies the pool's slabs [first, first + count) for the host walker (debug only).
+// ---------------------------------------------------------------------------------
+// Read-only find (the find<concurrent=false> batch path).  Each tile keeps K lookups in
+// flight: every iteration issues the K node (or record) loads back to back, then
+// advances each lookup by one node.  A wave of 64/T tiles therefore has 64K/T
+// independent 128-byte loads outstanding per iteration, which is what the latency-bound
+// descent needs.  Semantics are those of op_step's find path (layers, suffix records,
+// long values), with plain cached loads as no writer runs during the launch.
+constexpr u32 kFindDescend = 0, kFindRecord = 1, kFindIdle = 2;
+
+template <u32 T, u32 K>
+__global__ void __launch_bounds__(kBlockThreads) find_kernel(tree_view tv, batch_args a) {
+  wave_tile<T, false> c(tv.pool, tv.retire, tv.err);
+  const u32 tiles_per_block = kBlockThreads / T;
+  const u64 stride = (u64)gridDim.x * tiles_per_block * K;
+  const u64 first = ((u64)blockIdx.x * tiles_per_block + threadIdx.x / T) * K;
+  u64 idx[K];
+  u32 node[K], layer[K], phase[K], klen[K], kc[K], rkind[K];
+  const u32 maxK = a.max_key_length, maxV = a.max_value_length;
+
+  auto start = [&](u32 k) {
+    phase[k] = kFindIdle;
+    while (idx[k] < a.n) {
+      const u64 i = idx[k];
+      const u32 L = a.key_lengths ? a.key_lengths[i] : maxK;
+      if (L == 0 || L > maxK) {
+        c.set_error(kErrBadLength);
+        if (c.lane == 0) a.out_value_lengths[i] = 0;
+        idx[k] += stride;
+        continue;
+      }
+      klen[k] = L;
+      const u32* key = a.keys + i * (u64)maxK;
+      kc[k] = (c.lane < L) ? key[c.lane] : 0u;
+      node[k] = tv.root;
+      layer[k] = 0;
+      phase[k] = kFindDescend;
+      return;
+    }
+  };
+  auto finish = [&](u32 k, u32 len) {
+    if (c.lane == 0) a.out_value_lengths[idx[k]] = len;
+    idx[k] += stride;
+    start(k);
+  };
+
+#pragma unroll
+  for (u32 k = 0; k < K; k++) {
+    idx[k] = first + k;
+    start(k);
+  }
+  for (u32 guard = 0; guard < (1u << 24); guard++) {
+    bool any = false;
+#pragma unroll
+    for (u32 k = 0; k < K; k++) any |= phase[k] != kFindIdle;
+    if (!any) break;
+    // issue the K loads together
+    gpu_node n[K];
+#pragma unroll
+    for (u32 k = 0; k < K; k++) {
+      n[k].s = 0;
+      if (phase[k] != kFindIdle && node[k] < tv.pool.num_slabs)
+        n[k].s = c.slab(node[k])[c.slot];
+    }
+#pragma unroll
+    for (u32 k = 0; k < K; k++) {
+      if (phase[k] == kFindIdle) continue;
+      if (node[k] >= tv.pool.num_slabs) {
+        c.set_error(kErrBadIndex);
+        finish(k, 0);
+        continue;
+      }
+      const u64 i = idx[k];
+      const u32 l = layer[k];
+      const u32* key = a.keys + i * (u64)maxK;
+      if (phase[k] == kFindRecord) {
+        c.rec_slot = n[k].s;
+        c.rec_idx = node[k];
+        const u32 lw = c.bcast(slot_lo(n[k].s), 0);
+        request rq;
+        rq.key = key;
+        rq.key_length = klen[k];
+        rq.out_value = a.out_values + i * (u64)maxV;
+        rq.max_value_length = maxV;
+        const u32 saved = c.key_cache;
+        c.key_cache = kc[k];
+        bool hit = true;
+        if (rkind[k] == kSuffix) hit = c.rec_tail_equal(key, l + 1, klen[k] - l - 1, lw);
+        const u32 len = hit ? c.rec_copy_value(rq, lw) : 0u;
+        c.key_cache = saved;
+        finish(k, len);
+        continue;
+      }
+      const u32 q = l < T ? c.bcast(kc[k], l) : key[l];
+      const u32 h = c.header(n[k]);
+      if (!hdr_border(h)) {
+        u32 child = c.find_child(n[k], q);
+        node[k] = (child != kInvalid) ? child : c.sibling(n[k]);
+        continue;
+      }
+      if (q > c.high(n[k]) && c.sibling(n[k]) != kInvalid) {
+        node[k] = c.sibling(n[k]);
+        continue;
+      }
+      const bool more = l + 1 < klen[k];
+      u32 ks = 0, val = 0;
+      const int pos = c.lookup(n[k], q, more, ks, val);
+      if (pos < 0) {
+        finish(k, 0);
+      } else if (ks == kLink) {
+        layer[k] = l + 1;
+        node[k] = val;
+      } else if (ks == kValue) {
+        if (c.lane == 0 && maxV >= 1) a.out_values[i * (u64)maxV] = val;
+        finish(k, 1);
+      } else {
+        rkind[k] = ks;
+        node[k] = val;
+        phase[k] = kFindRecord;
+      }
+    }
+  }
+}
+
 }  // namespace indexingpu_amd

@@ -117,8 +117,8 @@ __device__ __forceinline__ bool claim_word(const slab_pool_view& v, bool leader,
   if (!leader) return true;
   const u32 tries = v.num_words + 1;
   for (u32 t = 0; t < tries; t++) {
-    u32 w = atomic_add_relaxed(v.cursor, 1u) % v.num_words;
-    u32 old = atomic_or_relaxed(&v.bitmap[w], 0xFFFFFFFFu);
+    u32 w = atomic_add(v.cursor, 1u) % v.num_words;
+    u32 old = atomic_or(&v.bitmap[w], 0xFFFFFFFFu);
     u32 fr = ~old;
     if (fr != 0) {
       word = w;
@@ -130,7 +130,7 @@ __device__ __forceinline__ bool claim_word(const slab_pool_view& v, bool leader,
 }
 
 __device__ __forceinline__ void free_slab(const slab_pool_view& v, u32 idx) {
-  atomic_and_relaxed(&v.bitmap[idx >> 5], ~(1u << (idx & 31)));
+  atomic_and(&v.bitmap[idx >> 5], ~(1u << (idx & 31)));
 }
 
 }  // namespace dev

@@ -28,7 +28,9 @@ struct gpu_node {
   u64 s;  // this lane's slot
 };
 
-template <u32 T>
+// Coherent = true when writers may run during the launch: node and record reads then
+// use agent-scope atomic (sc1) loads; read-only launches use plain cached loads.
+template <u32 T, bool Coherent = true>
 struct wave_tile {
   static_assert(T == 16 || T == 32 || T == 64, "tile width must be 16, 32 or 64 lanes");
   using node_t = gpu_node;
@@ -83,6 +85,10 @@ struct wave_tile {
     return false;
   }
   __device__ __forceinline__ u64* slab(u32 idx) const { return pool.pool + (u64)idx * kSlots; }
+  __device__ __forceinline__ u64 ld(const u64* p) const {
+    if (Coherent) return dev::coherent_load(p);
+    return *p;
+  }
   __device__ __forceinline__ u32* hdr_ptr(u32 idx) const {
     return reinterpret_cast<u32*>(slab(idx) + kMetaSlot);
   }
@@ -96,35 +102,67 @@ struct wave_tile {
       n.s = *p;
       return true;
     }
-    n.s = dev::load_relaxed(p);
+    // seqlock read: the whole line (header included) in one load, then the header again
+    n.s = dev::coherent_load(p);
     const u32 h = bcast(slot_lo(n.s), kMetaSlot);
     if (hdr_locked(h)) return false;
-    dev::fence_acquire();
-    const u32 h2 = dev::load_relaxed(hdr_ptr(idx));
+    dev::order();
+    const u32 h2 = dev::coherent_load(hdr_ptr(idx));
     return h2 == h;
   }
   __device__ __forceinline__ void read_node_locked(u32 idx, node_t& n) {
-    n.s = dev::load_relaxed(slab(idx) + slot);
+    n.s = dev::coherent_load(slab(idx) + slot);
   }
   __device__ __forceinline__ bool try_lock(u32 idx, u32& old) {
     if (!check(idx)) return false;
-    u32 o = 0;
-    if (lane == 0) o = dev::atomic_or_acq(hdr_ptr(idx), kLockBit);
+    // test-and-test-and-set: a locked header is seen with a load, so waiting tiles do
+    // not queue atomics in front of the holder's unlock
+    u32 o = kLockBit;
+    if (lane == 0) {
+      o = dev::coherent_load(hdr_ptr(idx));
+      if (!hdr_locked(o)) o = dev::atomic_or(hdr_ptr(idx), kLockBit);
+    }
     old = bcast(o, 0);
     if (hdr_locked(old)) return false;
-    dev::fence_release();
+    dev::order();
     return true;
   }
+  // Locks node idx only if its header still equals `expect` (the unlocked header of a
+  // validated read): kLockOk, kLockBusy (locked by another tile) or kLockStale (the
+  // node changed: re-read it).  The compare-and-swap is attempted only when a plain
+  // coherent load shows the expected header, so waiters do not queue atomics.
+  __device__ __forceinline__ u32 try_lock_snapshot(u32 idx, u32 expect) {
+    if (!check(idx)) return kLockStale;
+    u32 r = kLockStale;
+    if (lane == 0) {
+      u32* hp = hdr_ptr(idx);
+      u32 h = dev::coherent_load(hp);
+      if (h == expect) {
+        u32 e = expect;
+        if (__hip_atomic_compare_exchange_strong(hp, &e, expect | kLockBit, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+          r = kLockOk;
+        else
+          h = e;
+      }
+      if (r != kLockOk) r = hdr_locked(h) ? kLockBusy : kLockStale;
+    }
+    r = bcast(r, 0);
+    if (r == kLockOk) dev::order();
+    return r;
+  }
   __device__ __forceinline__ void unlock_same(u32 idx) {
-    if (lane == 0) dev::atomic_and_rel(hdr_ptr(idx), ~kLockBit);
+    dev::order();
+    if (lane == 0) dev::atomic_and(hdr_ptr(idx), ~kLockBit);
   }
   __device__ __forceinline__ void unlock_bump(u32 idx) {
-    if (lane == 0) dev::atomic_add_rel(hdr_ptr(idx), kVersionOne - kLockBit);
+    dev::order();  // the node's stores complete before the unlock publishes them
+    if (lane == 0) dev::atomic_add(hdr_ptr(idx), kVersionOne - kLockBit);
   }
   __device__ __forceinline__ void write_node(u32 idx, const node_t& n) {
-    if (lane < 16) dev::store_relaxed(slab(idx) + slot, n.s);
+    if (lane < 16) dev::coherent_store(slab(idx) + slot, n.s);
   }
-  __device__ __forceinline__ void fence_release() { dev::fence_release(); }
+  __device__ __forceinline__ void fence_release() { dev::order(); }
 
   // ---------------- node queries ----------------
   __device__ __forceinline__ u32 header(const node_t& n) const {
@@ -350,7 +388,7 @@ struct wave_tile {
     return ac.word * 32u + j;
   }
   __device__ __forceinline__ void flush() {
-    if (ac.bits != 0 && lane == 0) dev::atomic_and_relaxed(&pool.bitmap[ac.word], ~ac.bits);
+    if (ac.bits != 0 && lane == 0) dev::atomic_and(&pool.bitmap[ac.word], ~ac.bits);
     ac.bits = 0;
     if (rc.base != kInvalid) {
       const u32 pad = kRetireChunk - rc.used;
@@ -362,10 +400,10 @@ struct wave_tile {
     if (!rv.enabled) return;
     if (rc.base == kInvalid || rc.used == kRetireChunk) {
       u32 b = 0;
-      if (lane == 0) b = dev::atomic_add_relaxed(rv.count, kRetireChunk);
+      if (lane == 0) b = dev::atomic_add(rv.count, kRetireChunk);
       b = bcast(b, 0);
       if (b + kRetireChunk > rv.capacity) {
-        if (lane == 0) dev::atomic_add_relaxed(rv.leaked, 1u);
+        if (lane == 0) dev::atomic_add(rv.leaked, 1u);
         rc.base = kInvalid;
         return;
       }
@@ -380,14 +418,14 @@ struct wave_tile {
   __device__ __forceinline__ bool rec_load(u32 idx, u32& len_word) {
     if (!check(idx)) return false;
     rec_idx = idx;
-    rec_slot = slab(idx)[slot];
+    rec_slot = ld(slab(idx) + slot);
     len_word = bcast(slot_lo(rec_slot), 0);
     return true;
   }
   __device__ __forceinline__ bool rec_next(u64& cur) {
     const u32 nxt = bcast(slot_hi(cur), kMetaSlot);
     if (!check(nxt)) return false;
-    cur = slab(nxt)[slot];
+    cur = ld(slab(nxt) + slot);
     return true;
   }
   // compares the record tail with key[off, off + len); the record head is loaded
@@ -444,7 +482,7 @@ struct wave_tile {
       const u32 d0 = s * kRecordWordsPerSlab + 2 * slot, d1 = d0 + 1;
       const u32 v0 = word(d0, s, 0);
       const u32 v1 = (2 * slot + 1 == kRecordNextWord) ? nxt : word(d1, s, 1);
-      if (lane < 16) slab(cur)[slot] = slot_make(v0, v1);
+      if (lane < 16) dev::coherent_store(slab(cur) + slot, slot_make(v0, v1));
       cur = nxt;
     }
     return head;
@@ -494,7 +532,7 @@ struct wave_tile {
       if (d0 == 0) v0 = new_len_word;
       else if (d0 >= total) v0 = 0;
       if (slot < 15 && d1 >= total) v1 = 0;
-      if (lane < 16) slab(cur)[slot] = slot_make(v0, v1);
+      if (lane < 16) dev::coherent_store(slab(cur) + slot, slot_make(v0, v1));
       cur = nxt;
       // advance the old window by one slab
       oa = ob;

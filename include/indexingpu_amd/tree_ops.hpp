@@ -76,10 +76,38 @@ struct op_state {
   u32 a0, a1, a2, a3;  // split propagation: left, right, separator, parent level
   u32 result;      // find: value length; writes: 1 = success
   u32 steps;
+  u32 hint[4];     // last interior node visited at levels 1..4 of the current layer
   NodeImage img;   // last validated image of `node`
 };
 
-constexpr u32 kMaxStepsPerOp = 1u << 20;
+constexpr u32 kMaxStepsPerOp = 1u << 22;
+constexpr u32 kHintLevels = 4;
+
+// true while the operation holds node locks (a split waiting for its parent): such a
+// tile must retry at once rather than back off
+template <class S>
+IGPU_HD bool op_holds_locks(const S& st) {
+  return st.phase == ph_parent_descend || st.phase == ph_parent_lock;
+}
+
+template <class S>
+IGPU_HD void clear_hints(S& st) {
+  for (u32 k = 0; k < kHintLevels; k++) st.hint[k] = kInvalid;
+}
+
+// Continue a split at parent level `st.a3`: try the remembered node of that level
+// first (the B-link move-right in ph_parent_lock finds the parent if it split), else
+// descend from the layer root.
+template <class S>
+IGPU_HD void goto_parent(S& st) {
+  const u32 h = (st.a3 - 1 < kHintLevels) ? st.hint[st.a3 - 1] : kInvalid;
+  st.snap_hdr = kInvalid;
+  st.node = (h != kInvalid) ? h : st.layer_root;
+  st.phase = ph_parent_descend;
+}
+
+// results of try_lock_snapshot()
+constexpr u32 kLockOk = 0, kLockBusy = 1, kLockStale = 2;
 
 template <class C, class S>
 IGPU_HD void op_begin(C& c, S& st, u32 root) {
@@ -91,6 +119,7 @@ IGPU_HD void op_begin(C& c, S& st, u32 root) {
   st.a0 = st.a1 = st.a2 = st.a3 = 0;
   st.result = 0;
   st.steps = 0;
+  clear_hints(st);
   st.phase = ph_descend;
   (void)c;
 }
@@ -107,6 +136,7 @@ IGPU_HD void op_next_layer(S& st, u32 root) {
   st.node = root;
   st.layer_root = root;
   st.snap_hdr = kInvalid;
+  clear_hints(st);
   st.phase = ph_descend;
 }
 
@@ -145,6 +175,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       if (!c.template read_node<Validate>(st.node, n)) return false;
       const u32 h = c.header(n);
       if (!hdr_border(h)) {
+        if (hdr_level(h) - 1 < kHintLevels) st.hint[hdr_level(h) - 1] = st.node;
         u32 child = c.find_child(n, q);
         if (child == kInvalid) child = c.sibling(n);
         if (child == kInvalid) {
@@ -212,26 +243,20 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
     }
 
     case ph_lock: {
-      u32 old_hdr;
-      if (!c.try_lock(st.node, old_hdr)) return false;
-      node_t& n = st.img;
-      if (old_hdr != st.snap_hdr) c.read_node_locked(st.node, n);
-      // the in-memory header carries the lock bit while we hold the lock; keep the image
-      // identical so that write_node() never publishes an unlocked header
-      const u32 h = c.header(n) | kLockBit;
-      c.set_header(n, h);
-      if (!hdr_border(h)) {  // a layer root split in place since our read
-        c.unlock_same(st.node);
-        st.node = st.layer_root;
+      // Lock only the version we validated: a node that changed since is re-read
+      // optimistically (it may have split or become interior) instead of being locked
+      // just to find that out, which would serialize every waiter on a hot node.
+      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr);
+      if (r == kLockBusy) return false;
+      if (r == kLockStale) {
         st.phase = ph_descend;
         return true;
       }
-      if (q > c.high(n) && c.sibling(n) != kInvalid) {  // split since our read: move right
-        c.unlock_same(st.node);
-        st.node = c.sibling(n);
-        st.snap_hdr = kInvalid;
-        return true;
-      }
+      node_t& n = st.img;
+      // the in-memory header carries the lock bit while we hold the lock; keep the image
+      // identical so that write_node() never publishes an unlocked header
+      const u32 h = st.snap_hdr | kLockBit;
+      c.set_header(n, h);
       u32 ks = 0, val = 0;
       const int pos = c.lookup(n, q, more, ks, val);
       if (pos >= 0 && ks == kLink) {
@@ -316,9 +341,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         st.a2 = sep;
         st.a3 = 1;
         st.result = 1;
-        st.node = st.layer_root;
-        st.snap_hdr = kInvalid;
-        st.phase = ph_parent_descend;
+        goto_parent(st);
         return true;
       }
       // the entry (q, more) exists: VALUE / LONGVAL (key ends here) or SUFFIX
@@ -434,6 +457,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         st.phase = ph_parent_lock;
         return true;
       }
+      if (lvl - 1 < kHintLevels) st.hint[lvl - 1] = st.node;
       u32 child = c.find_child(n, st.a2);
       if (child == kInvalid) child = c.sibling(n);
       st.node = child;
@@ -441,29 +465,23 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
     }
 
     case ph_parent_lock: {
-      u32 old_hdr;
-      if (!c.try_lock(st.node, old_hdr)) return false;
-      node_t& n = st.img;
-      if (old_hdr != st.snap_hdr) c.read_node_locked(st.node, n);
-      const u32 h = c.header(n) | kLockBit;
-      c.set_header(n, h);
-      if (hdr_border(h) || hdr_level(h) != st.a3) {  // root grew since our read
-        c.unlock_same(st.node);
-        st.node = st.layer_root;
+      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr);
+      if (r == kLockBusy) return false;
+      if (r == kLockStale) {  // re-read the candidate (it may have split or grown)
         st.phase = ph_parent_descend;
         return true;
       }
+      node_t& n = st.img;
+      const u32 h = st.snap_hdr | kLockBit;
+      c.set_header(n, h);
       u32 j;
       if (!c.find_child_slot(n, st.a0, j)) {
+        // the separator's slot moved right with a split of this parent (or the parent
+        // is still being linked): continue with the right sibling, else from the root
         const u32 sib = c.sibling(n);
         c.unlock_same(st.node);
-        if (sib != kInvalid) {
-          st.node = sib;
-          st.snap_hdr = kInvalid;
-        } else {
-          st.node = st.layer_root;
-          st.phase = ph_parent_descend;
-        }
+        st.node = (sib != kInvalid) ? sib : st.layer_root;
+        st.phase = ph_parent_descend;
         return true;
       }
       if (hdr_nkeys(h) < kInteriorCap) {
@@ -526,9 +544,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       st.a1 = ri;
       st.a2 = sep2;
       st.a3 = st.a3 + 1;
-      st.node = st.layer_root;
-      st.snap_hdr = kInvalid;
-      st.phase = ph_parent_descend;
+      goto_parent(st);
       return true;
     }
 

@@ -33,6 +33,7 @@ struct index_base {
   virtual indexingpu_amd::walk_stats walk(const indexingpu_amd::entry_emit_fn&) = 0;
   virtual uint64_t slabs_allocated() = 0;
   virtual uint64_t pool_slabs() = 0;
+  virtual void set_find_depth(uint32_t k) = 0;
 };
 
 template <uint32_t T>
@@ -105,6 +106,7 @@ struct index_impl final : index_base {
   }
   uint64_t slabs_allocated() override { return alloc->num_allocated_slabs(); }
   uint64_t pool_slabs() override { return alloc->total_slabs(); }
+  void set_find_depth(uint32_t k) override { tree->set_find_depth(k); }
 };
 
 template <class F>
@@ -294,6 +296,12 @@ uint64_t gmt_slabs_allocated(gmt_index* index) {
 }
 
 uint64_t gmt_pool_slabs(gmt_index* index) { return index ? B(index)->pool_slabs() : 0; }
+
+int gmt_set_find_depth(gmt_index* index, uint32_t depth) {
+  if (!index) return bad("gmt_set_find_depth: null handle");
+  B(index)->set_find_depth(depth);
+  return GMT_OK;
+}
 
 uint32_t gmt_tile_size(gmt_index* index) { return index ? B(index)->tile() : 0; }
 

@@ -67,6 +67,7 @@ def load_library():
     lib.gmt_slabs_allocated.restype = u64
     lib.gmt_pool_slabs.argtypes = [vp]
     lib.gmt_pool_slabs.restype = u64
+    lib.gmt_set_find_depth.argtypes = [vp, u32]
     lib.gmt_tile_size.argtypes = [vp]
     lib.gmt_tile_size.restype = u32
     lib.gmt_last_error_message.restype = ctypes.c_char_p
@@ -212,6 +213,10 @@ class GpuMasstree:
         if n2 != n:
             raise MasstreeError("tree changed between dump calls")
         return ko, kd[:ks.value], vo, vd[:vs.value]
+
+    def set_find_depth(self, k):
+        """Lookups each tile keeps in flight in read-only find batches (1, 2, 4, 8)."""
+        self._check(self._lib.gmt_set_find_depth(self._h, int(k)))
 
     def slabs_allocated(self):
         return int(self._lib.gmt_slabs_allocated(self._h))

@@ -87,6 +87,20 @@ struct cpu_tile {
     __atomic_thread_fence(__ATOMIC_RELEASE);
     return true;
   }
+  u32 try_lock_snapshot(u32 idx, u32 expect) {
+    if (!check(idx)) return kLockStale;
+    u32 h = __atomic_load_n(hdr_ptr(idx), __ATOMIC_ACQUIRE);
+    if (h == expect) {
+      u32 e = expect;
+      if (__atomic_compare_exchange_n(hdr_ptr(idx), &e, expect | kLockBit, false,
+                                      __ATOMIC_ACQUIRE, __ATOMIC_RELAXED)) {
+        __atomic_thread_fence(__ATOMIC_RELEASE);
+        return kLockOk;
+      }
+      h = e;
+    }
+    return hdr_locked(h) ? kLockBusy : kLockStale;
+  }
   void unlock_same(u32 idx) { __atomic_fetch_and(hdr_ptr(idx), ~kLockBit, __ATOMIC_RELEASE); }
   void unlock_bump(u32 idx) {
     __atomic_fetch_add(hdr_ptr(idx), kVersionOne - kLockBit, __ATOMIC_RELEASE);
