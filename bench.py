@@ -66,6 +66,261 @@ def cpu_baseline(n_keys, seed):
                 seconds=t2 - t0)
 
 
+def run_mixed(torch, args):
+    """Mixed batches on the plain path: n preloaded 4-slice keys, 1M-op batches of Zipf
+    (0.99) finds / fresh inserts / erases of keys not read in the batch (90/5/5), then
+    65,536 range scans with uniform random lower bounds and limits in [1, 100].  Every
+    result of the timed batches is checked against the live-set model; scan results are
+    checked for order, bounds, limits and stored values."""
+    from indexingpu_amd import GpuMasstree, workloads as W
+    n = args.mixed_keys
+    log(f"[mixed] generating {n} 4-slice keys ...")
+    wl = W.MixedWorkload(n, key_len=4, max_value_length=2, theta=0.99, seed=args.seed + 1)
+    mk, mv = 4, 2
+    batches = [wl.batch(args.mixed_batch, check_fresh=False)
+               for _ in range(args.mixed_warmup + args.mixed_steps)]
+    tree = GpuMasstree(tile_size=args.tile, pool_bytes=int(args.mixed_pool_gb * (1 << 30)))
+    keys, lens, vals = up(torch, wl.keys), up(torch, wl.lens), up(torch, wl.values)
+    log("[mixed] preload ...")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    # grow the index in geometric steps (the step machine is fastest on a populated tree)
+    done = 0
+    step = 1 << 16
+    while done < n:
+        m = min(step, n - done)
+        tree.insert(keys[done * mk:(done + m) * mk], mk, lens[done:done + m],
+                    vals[done * mv:(done + m) * mv], mv, None, m)
+        done += m
+        step *= 4
+    torch.cuda.synchronize()
+    t_pre = time.perf_counter() - t0
+    tree.check_device_error()
+    del keys, lens, vals
+    dev = []
+    for b in batches:
+        dev.append(dict(types=up(torch, b["types"], np.uint8), keys=up(torch, b["keys"]),
+                        lens=up(torch, b["lens"]), values=up(torch, b["values"]),
+                        vlens=torch.full((len(b["types"]),), mv, dtype=torch.int32, device="cuda"),
+                        res=torch.zeros(len(b["types"]), dtype=torch.uint8, device="cuda")))
+    stream = torch.cuda.current_stream()
+    for b in dev[:args.mixed_warmup]:
+        tree.mixed_batch(b["types"], b["keys"], mk, b["lens"], b["values"], mv, b["vlens"],
+                         b["res"], len(b["lens"]))
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.mixed_steps)]
+    for (s, e), b in zip(ev, dev[args.mixed_warmup:]):
+        s.record(stream)
+        tree.mixed_batch(b["types"], b["keys"], mk, b["lens"], b["values"], mv, b["vlens"],
+                         b["res"], len(b["lens"]))
+        e.record(stream)
+    torch.cuda.synchronize()
+    tree.check_device_error()
+    per = [s.elapsed_time(e) for s, e in ev]
+    ok = True
+    for b, hb in zip(dev, batches):
+        vl = b["vlens"].cpu().numpy().view(np.uint32)
+        res = b["res"].cpu().numpy()
+        isf = hb["types"] == 2
+        vv = b["values"].cpu().numpy().view(np.uint32).reshape(-1, mv)
+        ok &= bool((vl[isf] == hb["expect_len"][isf]).all())
+        ok &= bool((vv[isf] == hb["expect_values"][isf]).all())
+        ok &= bool((res[~isf] == 1).all())
+    ops = args.mixed_batch * args.mixed_steps
+    mixed_mops = ops / (sum(per) / 1e3) / 1e6
+    # range scans
+    sc = wl.scans(args.scan_queries, max_scan=100)
+    nq = args.scan_queries
+    lo = up(torch, sc["lower"])
+    lim = up(torch, sc["limits"])
+    counts = torch.zeros(nq, dtype=torch.int32, device="cuda")
+    okeys = torch.zeros(nq * 100 * mk, dtype=torch.int32, device="cuda")
+    oklen = torch.zeros(nq * 100, dtype=torch.int32, device="cuda")
+    ovals = torch.zeros(nq * 100 * mv, dtype=torch.int32, device="cuda")
+    ovlen = torch.zeros(nq * 100, dtype=torch.int32, device="cuda")
+    tree.scan(lo, None, mk, 100, nq, counts=counts, values=ovals, max_value_length=mv,
+              value_lengths=ovlen, out_keys=okeys, out_key_lengths=oklen, per_query_limits=lim)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record(stream)
+    tree.scan(lo, None, mk, 100, nq, counts=counts, values=ovals, max_value_length=mv,
+              value_lengths=ovlen, out_keys=okeys, out_key_lengths=oklen, per_query_limits=lim)
+    e.record(stream)
+    torch.cuda.synchronize()
+    t_scan = s.elapsed_time(e)
+    tree.check_device_error()
+    c = counts.cpu().numpy().view(np.uint32)
+    kk = okeys.cpu().numpy().view(np.uint32).reshape(nq, 100, mk)
+    vv = ovals.cpu().numpy().view(np.uint32).reshape(nq, 100, mv)
+    # a lower bound within the last 100 keys of the key space returns fewer results
+    scan_ok = bool((c <= sc["limits"]).all()) and float((c == sc["limits"]).mean()) > 0.999
+    for q in range(0, nq, max(1, nq // 512)):
+        rows = [tuple(r) for r in kk[q, :c[q]].tolist()]
+        scan_ok &= rows == sorted(rows) and len(set(rows)) == len(rows)
+        scan_ok &= (not rows) or rows[0] >= tuple(sc["lower"][q].tolist())
+        if rows:
+            exp = W.make_values(kk[q, :c[q]], np.full(c[q], mk, np.uint32), mv)
+            scan_ok &= bool((vv[q, :c[q]] == exp).all())
+    tree.close()
+    return dict(mixed_mops=round(mixed_mops, 2), mixed_ms_per_batch=round(float(np.mean(per)), 3),
+                mixed_batch=args.mixed_batch, mixed_preload_keys=n,
+                mixed_preload_mops=round(n / t_pre / 1e6, 2), mixed_verified=ok,
+                scan_queries=nq, scan_ms=round(t_scan, 3),
+                scan_kqps=round(nq / t_scan, 1), scan_results_mops=round(float(c.sum()) / t_scan / 1e3, 2),
+                scan_verified=scan_ok)
+
+
+def _preload(torch, tree, keys, lens, vals, mk, mv):
+    """Inserts the preload in geometric steps; returns Mops/s."""
+    n = len(lens)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done, step = 0, 1 << 16
+    while done < n:
+        m = min(step, n - done)
+        tree.insert(keys[done:done + m], mk, lens[done:done + m], vals[done:done + m], mv,
+                    None, m)
+        done += m
+        step *= 4
+    torch.cuda.synchronize()
+    tree.check_device_error()
+    return n / (time.perf_counter() - t0) / 1e6
+
+
+def run_batched(torch, args):
+    """Batched form: keys of 16 slices whose first 12 are one of 256 random prefixes,
+    args.batched_keys preloaded, then args.batched_count uneven mixed batches
+    (floor(exp(U(ln 1K, ln 1M))) ops, 90% Zipf finds / 5% fresh inserts / 5% erases of
+    live keys not read in the batch).  The same batch sequence runs on a freshly
+    preloaded index in four ways:
+      plain  - one mixed_batch call per batch, launched eagerly;
+      graph  - the same calls captured once into a hipGraph and replayed;
+      multi  - one batched-form launch over all batches (tile 16, full grid);
+      tuned  - the batched form with the autotuner's tile width and grid share, tuned
+               beforehand on a separate workload of the same distribution.
+    Every request's output of every way is checked against the generator's model."""
+    from indexingpu_amd import GpuMasstree
+    from indexingpu_amd import device_workloads as D
+    from indexingpu_amd.autotune import Autotuner
+    n = args.batched_keys
+    pool = int(args.batched_pool_gb * (1 << 30))
+    log(f"[batched] generating {n} preload keys and {args.batched_count} batches ...")
+    wl = D.PrefixBatchWorkload(n, seed=args.seed + 7)
+    mk, mv = wl.key_len, wl.mv
+    pkeys, plens, pvals = wl.preload()
+    rng = np.random.default_rng(args.seed + 8)
+    sizes = D.log_uniform_sizes(rng, args.batched_count, 1024, 1 << 20)
+    cat = D.concat_batches([wl.batch(int(s)) for s in sizes])
+    total = len(cat["types"])
+    offs = cat["offsets"]
+    torch.cuda.synchronize()
+    log(f"[batched] {total} requests in {len(sizes)} batches")
+
+    def fresh_outputs():
+        return (cat["values"].clone(), torch.full((total,), mv, dtype=torch.int32, device="cuda"),
+                torch.zeros(total, dtype=torch.uint8, device="cuda"))
+
+    def verified(v, vl, r):
+        return (torch.equal(vl, cat["expect_len"]) and torch.equal(v, cat["expect_values"])
+                and torch.equal(r, cat["expect_results"]))
+
+    def per_batch_calls(tree, v, vl, r):
+        o = offs.tolist()
+        for i in range(len(sizes)):
+            a, b = o[i], o[i + 1]
+            tree.mixed_batch(cat["types"][a:b], cat["keys"][a:b], mk, cat["lens"][a:b],
+                             v[a:b], mv, vl[a:b], r[a:b], b - a)
+
+    def multi_call(tree, v, vl, r):
+        tree.multi_batch(cat["types"], cat["keys"], mk, cat["lens"], v, mv, vl, r, offs,
+                         len(sizes), total)
+
+    stream = torch.cuda.current_stream()
+
+    def timed_ms(fn):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        s.record(stream)
+        fn()
+        e.record(stream)
+        torch.cuda.synchronize()
+        return s.elapsed_time(e)
+
+    # autotune the batched form on a separate workload of the same distribution
+    log("[batched] tuning tile width x grid share ...")
+    twl = D.PrefixBatchWorkload(n, seed=args.seed + 9)
+    tk, tl, tv = twl.preload()
+    ttree = GpuMasstree(tile_size=16, pool_bytes=pool)
+    _preload(torch, ttree, tk, tl, tv, mk, mv)
+    del tk, tl, tv
+    tsizes = iter(D.log_uniform_sizes(rng, 4096, 1024, 1 << 20))
+    per_unit = args.batched_tune_unit
+
+    def tune_units():
+        units = []
+        for _ in range(3 * 3 * 2):
+            c = D.concat_batches([twl.batch(int(next(tsizes))) for _ in range(per_unit)])
+            units.append(c)
+        return iter(units)
+
+    units = tune_units()
+    torch.cuda.synchronize()
+
+    def run_unit(cfg):
+        c = next(units)
+        m = len(c["types"])
+        v = c["values"].clone()
+        vl = torch.full((m,), mv, dtype=torch.int32, device="cuda")
+        r = torch.zeros(m, dtype=torch.uint8, device="cuda")
+        ttree.multi_batch(c["types"], c["keys"], mk, c["lens"], v, mv, vl, r, c["offsets"],
+                          len(c["sizes"]), m)
+        return m
+
+    tuner = Autotuner(ttree, trials=2)
+    best = tuner.tune_online("multi", run_unit, mk)
+    ttree.check_device_error()
+    tune_table = [{"tile": c["tile"], "grid_percent": c["grid_percent"],
+                   "mops": round(1e3 / ms / 1e6, 2) if ms > 0 else None}
+                  for _, _, c, ms in tuner.report()]
+    ttree.close()
+    del units
+    torch.cuda.empty_cache()
+    log(f"[batched] tuned: {best}")
+
+    out = {}
+    for way in ("plain", "graph", "multi", "tuned"):
+        tree = GpuMasstree(tile_size=16, pool_bytes=pool)
+        pre = _preload(torch, tree, pkeys, plens, pvals, mk, mv)
+        v, vl, r = fresh_outputs()
+        if way == "plain":
+            ms = timed_ms(lambda: per_batch_calls(tree, v, vl, r))
+        elif way == "graph":
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                per_batch_calls(tree, v, vl, r)
+            ms = timed_ms(g.replay)
+        elif way == "multi":
+            ms = timed_ms(lambda: multi_call(tree, v, vl, r))
+        else:
+            tree.tile_size = best["tile"]
+            tree.set_multi_grid_percent(best["grid_percent"])
+            ms = timed_ms(lambda: multi_call(tree, v, vl, r))
+        tree.check_device_error()
+        ok = verified(v, vl, r)
+        out[way] = dict(mops=round(total / ms / 1e3, 2), ms=round(ms, 2), verified=ok,
+                        preload_mops=round(pre, 2))
+        log(f"[batched] {way}: {total / ms / 1e3:.1f} Mops/s, verified {ok}")
+        if way == "graph":
+            del g
+        tree.close()
+        del v, vl, r
+        torch.cuda.empty_cache()
+    return dict(batched=dict(preload_keys=n, batches=len(sizes), requests=total,
+                             key_slices=mk, prefixes=256, tuned=best, tune_table=tune_table,
+                             **out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +332,22 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed", action="store_true", help="also run 64M-key mixed batches")
+    ap.add_argument("--mixed-keys", type=int, default=1 << 26)
+    ap.add_argument("--mixed-batch", type=int, default=1 << 20)
+    ap.add_argument("--mixed-steps", type=int, default=8)
+    ap.add_argument("--mixed-warmup", type=int, default=2)
+    ap.add_argument("--mixed-pool-gb", type=float, default=48.0)
+    ap.add_argument("--scan-queries", type=int, default=65536)
+    ap.add_argument("--batched", action="store_true",
+                    help="also run the batched-form comparison (16-slice prefixed keys)")
+    ap.add_argument("--batched-keys", type=int, default=1 << 26)
+    ap.add_argument("--batched-count", type=int, default=2000)
+    ap.add_argument("--batched-pool-gb", type=float, default=32.0)
+    ap.add_argument("--batched-tune-unit", type=int, default=16,
+                    help="batches per autotuner measurement unit")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (a kernel of "
+                         "known traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--seed", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -88,6 +359,13 @@ def main():
     import torch
     from indexingpu_amd import GpuMasstree, workloads as W
 
+    if args.calibrate_copy_mib:
+        src = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     log(f"[bench] generating {args.keys} keys ...")
     t0 = time.time()
     d = W.plain_batch_workload(args.keys, seed=args.seed)
@@ -191,6 +469,11 @@ def main():
     tree.close()
     del keys, lens, vals, fkeys, flens, ekeys, elens, out_v, out_l
     torch.cuda.empty_cache()
+
+    if args.mixed:
+        result.update(run_mixed(torch, args))
+    if args.batched:
+        result.update(run_batched(torch, args))
 
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline: oracle on {args.cpu_sample} keys ...")

@@ -98,6 +98,20 @@ int gmt_mixed_batch(gmt_index* index, const uint8_t* types, const uint32_t* keys
                     uint32_t max_value_length, uint32_t* value_lengths, uint8_t* results,
                     uint32_t num_requests, void* stream);
 
+/* Batched form (no counterpart in the reference; it extends mixed_batch): applies
+ * num_batches mixed batches in order within one persistent launch.  Requests of all
+ * batches are stored back to back with gmt_mixed_batch's layout; `offsets` is a device
+ * array of num_batches + 1 request offsets; total_requests = offsets[num_batches]. */
+int gmt_multi_batch(gmt_index* index, const uint8_t* types, const uint32_t* keys,
+                    uint32_t max_key_length, const uint32_t* key_lengths, uint32_t* values,
+                    uint32_t max_value_length, uint32_t* value_lengths, uint8_t* results,
+                    const uint64_t* offsets, uint32_t num_batches, uint64_t total_requests,
+                    void* stream);
+
+/* Tuning knob of gmt_multi_batch: share (1..100 %) of the resident-block capacity the
+ * persistent launch uses. */
+int gmt_set_multi_grid_percent(gmt_index* index, uint32_t percent);
+
 /* Waits for the device and returns (then clears, if clear != 0) the first device-side
  * error code since the last clear (gmt_device_error_code). */
 uint32_t gmt_device_error(gmt_index* index, int clear);
@@ -124,7 +138,12 @@ uint64_t gmt_pool_slabs(gmt_index* index);
  * values select 4).  A tuning knob with no counterpart in the reference. */
 int gmt_set_find_depth(gmt_index* index, uint32_t depth);
 
-/* Tile width the index was created with. */
+/* Tile width (lanes per operation: 16, 32 or 64) that later calls on this index use.
+ * The reference fixes it per instance as the tile_size template argument
+ * (include/gpu_masstree.hpp:42-46); here all widths share one node layout and one tree
+ * state, so it can change between calls (a tuning knob).  gmt_tile_size returns the
+ * current width. */
+int gmt_set_tile_size(gmt_index* index, uint32_t tile_size);
 uint32_t gmt_tile_size(gmt_index* index);
 
 /* Message of the last failing call on this thread. */

@@ -128,6 +128,15 @@ struct gpu_masstree {
         reclaimer_(other.reclaimer_),
         err_(other.err_),
         owner_(false) {}
+  // Attaches to the tree of an index with another tile width: the node layout does not
+  // depend on the tile width, so both operate on the same tree (the owner frees it).
+  template <uint32_t other_tile>
+  explicit gpu_masstree(const gpu_masstree<Allocator, Reclaimer, other_tile>& other)
+      : root_index_(other.root_index_),
+        allocator_(other.allocator_),
+        reclaimer_(other.reclaimer_),
+        err_(other.err_),
+        owner_(false) {}
   gpu_masstree& operator=(const gpu_masstree&) = delete;
   ~gpu_masstree() {
     if (owner_) {
@@ -261,6 +270,42 @@ struct gpu_masstree {
     a.results = results;
     launch_batch<true>(a, stream, true);
   }
+
+  // Batched form (no counterpart in the reference): applies `num_batches` mixed batches
+  // in order within one persistent launch.  The arrays hold the requests of every batch
+  // back to back, with mixed_batch's layout; `offsets` is a device array of
+  // num_batches + 1 u64 request offsets.  Same per-request outputs as mixed_batch.
+  void multi_batch(const kernels::request_type* request_types, const key_slice_type* keys,
+                   const size_type max_key_length, const size_type* key_lengths,
+                   value_slice_type* values, const size_type max_value_length,
+                   size_type* value_lengths, bool* results, const uint64_t* offsets,
+                   const size_type num_batches, const uint64_t total_requests,
+                   hipStream_t stream = 0) {
+    if (num_batches == 0) return;
+    indexingpu_amd::multi_args m{};
+    m.a = base_args(keys, max_key_length, key_lengths, total_requests);
+    m.a.types = reinterpret_cast<const uint8_t*>(request_types);
+    m.a.kind = indexingpu_amd::op_find;
+    m.a.values = values;
+    m.a.max_value_length = max_value_length;
+    m.a.value_lengths = value_lengths;
+    m.a.out_values = values;
+    m.a.out_value_lengths = value_lengths;
+    m.a.results = results;
+    m.offsets = offsets;
+    m.num_batches = num_batches;
+    m.barrier = err_ + 2;
+    auto kernel = indexingpu_amd::multi_batch_kernel<tile_size>;
+    // every block must be resident at once (the kernel has a grid-wide barrier)
+    const uint32_t blocks = std::max<uint32_t>(
+        1, (uint32_t)((uint64_t)indexingpu_amd::resident_blocks(kernel) * multi_grid_percent_ / 100));
+    hipLaunchKernelGGL(indexingpu_amd::zero_kernel, dim3(1), dim3(64), 0, stream, m.barrier, 1u);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
+                       view(), m);
+    drain(stream);
+  }
+  // Fraction (percent) of the resident-block capacity the batched form launches.
+  void set_multi_grid_percent(uint32_t p) { multi_grid_percent_ = std::min<uint32_t>(std::max<uint32_t>(p, 1), 100); }
 
   // ---------------------------------------------------------------- device API
   // Every lane of the tile calls these with the same arguments.  Keys and values are
@@ -486,6 +531,7 @@ struct gpu_masstree {
 
   bool owner_ = true;
   uint32_t find_depth_ = 4;
+  uint32_t multi_grid_percent_ = 100;
   uint32_t* scan_stack_ = nullptr;
   uint64_t scan_stack_words_ = 0;
 };

@@ -99,7 +99,7 @@ class host_walker {
       level_heads.push_back(cur);
       if (hdr_border(hh)) break;
       if (hdr_nkeys(hh) == 0) return fail("empty interior node");
-      cur = slot_hi(p[0]);
+      cur = child_of(p[0]);
       if (level == 0) return fail("interior node at level 0");
       level--;
     }
@@ -117,11 +117,14 @@ class host_walker {
         if (hdr_border(hh)) return fail("border node in interior level");
         uint32_t nk = hdr_nkeys(hh);
         if (nk == 0 || nk > kInteriorCap) return fail("bad interior nkeys");
+        uint32_t tag = 0;
+        for (uint32_t i = 0; i < kTagSlots; i++) tag |= (uint32_t)(p[i] >> 63) << i;
+        if (tag != hdr_tag(hh)) return fail("interior version tag mismatch");
         for (uint32_t i = 0; i < nk; i++) {
           uint32_t b = slot_lo(p[i]);
           if (i > 0 && b <= slot_lo(p[i - 1])) return fail("interior bounds not ascending");
           if (!first && i == 0 && b <= prev_high) return fail("interior bound below left high");
-          const uint64_t* c = node(slot_hi(p[i]));
+          const uint64_t* c = node(child_of(p[i]));
           if (!c) return false;
           if (hdr_level(slot_lo(c[kMetaSlot])) + 1 != hdr_level(hh))
             return fail("child level mismatch");

@@ -132,6 +132,144 @@ __global__ void __launch_bounds__(kBlockThreads) batch_kernel(tree_view tv, batc
 }
 
 // ---------------------------------------------------------------------------------
+// Batched form: one persistent launch applies a sequence of mixed batches in order
+// (batch b + 1 sees every effect of batch b).  Requests of all batches are concatenated;
+// offsets[b] .. offsets[b + 1] delimit batch b.  Tiles take the requests of the current
+// batch by static stride; a tile with none left reports to its block (LDS counter), the
+// last tile of the block adds one to a global counter, and a leader tile per block polls
+// that counter and releases the block into the next batch.  The barrier lives in the
+// same flat loop as the operations, so waiting tiles never stall working sibling tiles
+// of their wave.  Records retired by any batch are drained once, after the launch.
+struct multi_args {
+  batch_args a;         // a.n = total requests of all batches
+  const u64* offsets;   // num_batches + 1 request offsets
+  u32 num_batches;
+  u32* barrier;         // zero at launch; blocks add 1 per finished batch
+};
+
+constexpr u32 kBarrierGuard = 1u << 26;
+
+template <u32 T>
+__global__ void __launch_bounds__(kBlockThreads) multi_batch_kernel(tree_view tv, multi_args m) {
+  __shared__ u32 s_done;      // tiles of this block finished with the current batch
+  __shared__ u32 s_released;  // batches known complete grid-wide
+  if (threadIdx.x == 0) {
+    s_done = 0;
+    s_released = 0;
+  }
+  __syncthreads();
+  wave_tile<T, true> c(tv.pool, tv.retire, tv.err);
+  const batch_args& a = m.a;
+  const u32 tiles_per_block = kBlockThreads / T;
+  const u64 ntiles = (u64)gridDim.x * tiles_per_block;
+  const u32 tile_in_block = threadIdx.x / T;
+  const u64 gtile = (u64)blockIdx.x * tiles_per_block + tile_in_block;
+  op_state<gpu_node> st;
+  request rq;
+  u32 b = 0;
+  // offsets are clamped to a.n so a malformed offsets array cannot index past the
+  // request arrays
+  u64 i = min(m.offsets[0], a.n) + gtile;
+  u64 end = min(m.offsets[1], a.n);
+  bool have = i < end;
+  bool marked = false;
+  u32 backoff = 0, wait = 0, idle = 0;
+  auto start = [&]() {
+    load_request<T>(a, i, rq);
+    const bool writes = rq.kind == op_insert || rq.kind == op_update || rq.kind == op_upsert;
+    const bool bad_len = rq.key_length == 0 || rq.key_length > a.max_key_length ||
+                         (writes && rq.value_length > a.max_value_length);
+    if (bad_len) rq.key_length = 0;
+    c.load_key_cache(rq);
+    op_begin(c, st, tv.root);
+    if (bad_len) {
+      c.set_error(kErrBadLength);
+      op_finish(st, 0);
+    } else if (abort_pending(tv.err)) {
+      op_finish(st, 0);
+    }
+  };
+  if (have) start();
+  while (b < m.num_batches) {
+    bool progressed = true;
+    if (have) {
+      if (wait > 0) {
+        wait -= 1;
+        st.steps += 1;
+        progressed = false;
+      } else if (st.phase != ph_done) {
+        progressed = op_step<true>(c, rq, st);
+        if (progressed || op_holds_locks(st)) {
+          backoff = 0;
+        } else {
+          if (backoff < kMaxBackoffShift) backoff += 1;
+          const u32 span = 1u << backoff;
+          const u32 jitter = (u32)(i * 0x9E3779B1u) ^ (st.steps * 0x85EBCA6Bu);
+          wait = span - 1 + ((jitter >> 7) & (span - 1));
+        }
+      }
+      if (st.phase == ph_done) {
+        backoff = 0;
+        wait = 0;
+        if (c.lane == 0) {
+          if (rq.kind == op_find) {
+            if (a.out_value_lengths) a.out_value_lengths[i] = st.result;
+          } else if (a.results) {
+            a.results[i] = st.result != 0;
+          }
+        }
+        i += ntiles;
+        have = i < end;
+        if (have) start();
+      }
+    } else {
+      progressed = false;
+      if (!marked) {
+        marked = true;
+        idle = 0;
+        u32 v = 0;
+        if (c.lane == 0) v = atomicAdd(&s_done, 1u) + 1u;
+        v = c.bcast(v, 0);
+        if (v == tiles_per_block && c.lane == 0) {
+          s_done = 0;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          dev::order();
+          dev::atomic_add(m.barrier, 1u);
+        }
+      }
+      if (tile_in_block == 0) {
+        u32 g = 0;
+        if (c.lane == 0) g = dev::coherent_load(m.barrier);
+        g = c.bcast(g, 0);
+        if (g >= (b + 1) * gridDim.x && c.lane == 0) s_released = b + 1;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const u32 rel = c.bcast(*((volatile u32*)&s_released), 0);
+      if (rel > b) {
+        b += 1;
+        marked = false;
+        if (b < m.num_batches) {
+          i = min(m.offsets[b], a.n) + gtile;
+          end = min(m.offsets[b + 1], a.n);
+          have = i < end;
+          if (have) start();
+        }
+        progressed = true;
+      } else if (++idle > kBarrierGuard) {  // a block never arrived: give up, do not hang
+        c.set_error(kErrStepLimit);
+        break;
+      }
+    }
+    if (__ballot(!progressed) == __ballot(true)) __builtin_amdgcn_s_sleep(1);
+  }
+  c.flush();
+}
+
+__global__ void zero_kernel(u32* p, u32 n) {
+  for (u32 k = threadIdx.x; k < n; k += blockDim.x) p[k] = 0;
+}
+
+// ---------------------------------------------------------------------------------
 // Range scan (read-only).  Outputs per query q at slot q * max_count + c: values
 // (max_value_length slices), value lengths, keys (max_key_length slices), key
 // lengths; counts[q].  Semantics follow the reference's cooperative_scan: ascending

@@ -94,6 +94,23 @@ IGPU_HD bool entry_less(u32 s1, bool m1, u32 s2, bool m2) {
   return s1 < s2 || (s1 == s2 && !m1 && m2);
 }
 
+// Interior-node version tag.  Bit 31 of the child index in slots 0..7 (the first 64 bytes
+// of the line; slab indices stay below 2^31) holds bit i of the low 8 bits of the version
+// the node has once its writer releases it; the header (slot 15) is in the other half.
+// A reader may take an interior node from a plain cached load if the header is unlocked
+// and the tag matches: a line whose halves come from different versions fails the check,
+// while an old but whole line is harmless to a B-link descent (keys only move right).
+constexpr u32 kTagSlots = 8;
+constexpr u32 kChildMask = 0x7FFFFFFFu;
+constexpr u32 kMaxPoolSlabs = 0x7FFFFFE0u;
+// version a node will carry once published: writers store locked nodes and release them
+// with a version increment; nodes stored unlocked keep their version
+IGPU_HD u32 hdr_publish_tag(u32 h) {
+  return ((h >> kVersionShift) + (hdr_locked(h) ? 1u : 0u)) & 0xFFu;
+}
+IGPU_HD u32 hdr_tag(u32 h) { return (h >> kVersionShift) & 0xFFu; }
+IGPU_HD u32 child_of(u64 s) { return (u32)(s >> 32) & kChildMask; }
+
 IGPU_HD u32 rec_len_word(u32 tail_len, u32 value_len) { return (value_len << 16) | tail_len; }
 IGPU_HD u32 rec_tail_len(u32 w) { return w & 0xFFFFu; }
 IGPU_HD u32 rec_value_len(u32 w) { return w >> 16; }

@@ -80,7 +80,8 @@ class simple_slab_allocator {
   void init(std::size_t bytes) {
     uint64_t slabs = bytes / kSlabBytes;
     slabs = slabs / 32 * 32;  // whole bitmap words
-    if (slabs > 0xFFFFFFE0ull) slabs = 0xFFFFFFE0ull;  // u32 slab indices, kInvalid reserved
+    // slab indices stay below 2^31: bit 31 of an interior child index is the version tag
+    if (slabs > kMaxPoolSlabs) slabs = kMaxPoolSlabs;
     if (slabs < 64) slabs = 64;
     view_.num_slabs = (u32)slabs;
     view_.num_words = (u32)(slabs / 32);

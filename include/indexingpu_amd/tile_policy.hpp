@@ -94,13 +94,22 @@ struct wave_tile {
   }
 
   // ---------------- node reads / locking ----------------
+  // cached_ok: the node is expected to be interior; it is first read with a plain cached
+  // load and accepted if unlocked with a matching version tag (see layout.hpp), which
+  // keeps the hot upper levels of the tree in L1/L2 instead of one memory channel.
   template <bool Validate>
-  __device__ __forceinline__ bool read_node(u32 idx, node_t& n) {
+  __device__ __forceinline__ bool read_node(u32 idx, node_t& n, bool cached_ok = false) {
     if (!check(idx)) return false;
     const u64* p = slab(idx) + slot;
     if (!Validate) {
       n.s = *p;
       return true;
+    }
+    if (cached_ok) {
+      n.s = *p;
+      const u32 h = bcast(slot_lo(n.s), kMetaSlot);
+      const u32 tag = ballot(slot < kTagSlots && (slot_hi(n.s) >> 31) != 0) & 0xFFu;
+      if (!hdr_border(h) && !hdr_locked(h) && tag == hdr_tag(h)) return true;
     }
     // seqlock read: the whole line (header included) in one load, then the header again
     n.s = dev::coherent_load(p);
@@ -160,7 +169,13 @@ struct wave_tile {
     if (lane == 0) dev::atomic_add(hdr_ptr(idx), kVersionOne - kLockBit);
   }
   __device__ __forceinline__ void write_node(u32 idx, const node_t& n) {
-    if (lane < 16) dev::coherent_store(slab(idx) + slot, n.s);
+    const u32 h = bcast(slot_lo(n.s), kMetaSlot);
+    u64 s = n.s;
+    if (!hdr_border(h) && slot < kTagSlots) {
+      const u64 bit = (u64)((hdr_publish_tag(h) >> slot) & 1u) << 63;
+      s = (s & ~(1ull << 63)) | bit;
+    }
+    if (lane < 16) dev::coherent_store(slab(idx) + slot, s);
   }
   __device__ __forceinline__ void fence_release() { dev::order(); }
 
@@ -189,12 +204,12 @@ struct wave_tile {
   __device__ __forceinline__ u32 find_child(const node_t& n, u32 q) const {
     const u32 nk = nkeys_of(n);
     const u32 m = ballot(slot < nk && q <= slot_lo(n.s));
-    const u32 child = bcast(slot_hi(n.s), m ? (u32)__builtin_ctz(m) : 0u);
+    const u32 child = bcast(slot_hi(n.s), m ? (u32)__builtin_ctz(m) : 0u) & kChildMask;
     return m ? child : kInvalid;
   }
   __device__ __forceinline__ bool find_child_slot(const node_t& n, u32 child, u32& j) const {
     const u32 nk = nkeys_of(n);
-    const u32 m = ballot(slot < nk && slot_hi(n.s) == child);
+    const u32 m = ballot(slot < nk && child_of(n.s) == child);
     j = m ? (u32)__builtin_ctz(m) : 0u;
     return m != 0;
   }

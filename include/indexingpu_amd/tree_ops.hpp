@@ -77,6 +77,7 @@ struct op_state {
   u32 result;      // find: value length; writes: 1 = success
   u32 steps;
   u32 hint[4];     // last interior node visited at levels 1..4 of the current layer
+  u32 lvl;         // expected level of `node` (kInvalid: unknown)
   NodeImage img;   // last validated image of `node`
 };
 
@@ -93,6 +94,7 @@ IGPU_HD bool op_holds_locks(const S& st) {
 template <class S>
 IGPU_HD void clear_hints(S& st) {
   for (u32 k = 0; k < kHintLevels; k++) st.hint[k] = kInvalid;
+  st.lvl = kInvalid;
 }
 
 // Continue a split at parent level `st.a3`: try the remembered node of that level
@@ -172,20 +174,29 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
   switch (st.phase) {
     case ph_descend: {
       node_t& n = st.img;
-      if (!c.template read_node<Validate>(st.node, n)) return false;
+      // interior nodes (expected level > 0, or unknown at a layer root) may come from
+      // the cache; border nodes are always read coherently, since writers lock them by
+      // the version they read
+      if (!c.template read_node<Validate>(st.node, n, st.lvl != 0)) return false;
       const u32 h = c.header(n);
       if (!hdr_border(h)) {
         if (hdr_level(h) - 1 < kHintLevels) st.hint[hdr_level(h) - 1] = st.node;
         u32 child = c.find_child(n, q);
-        if (child == kInvalid) child = c.sibling(n);
+        u32 next_level = hdr_level(h) - 1;
+        if (child == kInvalid) {
+          child = c.sibling(n);
+          next_level = hdr_level(h);
+        }
         if (child == kInvalid) {
           c.set_error(kErrBadIndex);
           op_finish(st, 0);
           return true;
         }
         st.node = child;
+        st.lvl = next_level;
         return true;
       }
+      st.lvl = 0;
       if (q > c.high(n) && c.sibling(n) != kInvalid) {
         st.node = c.sibling(n);
         return true;

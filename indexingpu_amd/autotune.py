@@ -1,0 +1,153 @@
+"""Autotuner for the index's run-time knobs, per operation class and key-length class.
+
+The knobs (all switchable between calls on one index, same tree state):
+  * tile width: lanes per operation, 16 / 32 / 64 (the reference's tile_size template
+    argument, include/gpu_masstree.hpp:42-46, fixed per instance there);
+  * find depth: lookups each tile keeps in flight in read-only find batches (1/2/4/8);
+  * grid share: percent of the resident-block capacity the batched form launches.
+
+Node fanout and slab size class are not knobs here: every node is one 128-byte slab of
+16 eight-byte slots (border nodes hold up to 14 keys, interior nodes up to 15 children),
+the size one 16-lane tile reads in one coalesced access, and records use the same slab
+class.  The tuner therefore searches tile x depth (finds) and tile x grid share
+(batched form).
+
+Choices are cached per (operation class, key-length class): the key-length class is the
+maximum key length rounded up to a power of two (1, 2, 4, 8, 16, 32, 64, ...), since the
+length decides how many trie layers a lookup walks and how much key data each tile
+streams.
+
+Two tuning modes:
+  * tune_find: read-only and repeatable, so every candidate runs the same batch;
+  * tune_online: for mutating work every candidate runs fresh units of the same
+    distribution (``run_unit(config)`` draws and runs the next unit), so no operation is
+    applied twice.
+"""
+import itertools
+import time
+
+TILES = (16, 32, 64)
+FIND_DEPTHS = (1, 2, 4, 8)
+GRID_PERCENTS = (100, 50, 25)
+
+
+def length_class(max_key_length):
+    """Key-length class: max key length rounded up to a power of two."""
+    m = max(1, int(max_key_length))
+    return 1 << (m - 1).bit_length()
+
+
+class _EventTimer:
+    """Elapsed milliseconds of fn() on torch's current stream (HIP events)."""
+
+    def __init__(self):
+        import torch
+        self.torch = torch
+
+    def __call__(self, fn):
+        torch = self.torch
+        stream = torch.cuda.current_stream()
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        fn()
+        e.record(stream)
+        e.synchronize()
+        return s.elapsed_time(e)
+
+
+class WallTimer:
+    """Elapsed milliseconds by the host clock (for callers without a GPU stream)."""
+
+    def __call__(self, fn):
+        t = time.perf_counter()
+        fn()
+        return (time.perf_counter() - t) * 1e3
+
+
+class Autotuner:
+    """Picks knob settings for an index by measurement and applies them on request.
+
+    ``index`` needs ``tile_size`` (settable), ``set_find_depth`` and
+    ``set_multi_grid_percent`` -- the GpuMasstree wrapper has all three.
+    ``timer(fn) -> ms`` defaults to HIP events on torch's current stream.
+    """
+
+    def __init__(self, index, tiles=TILES, find_depths=FIND_DEPTHS,
+                 grid_percents=GRID_PERCENTS, trials=3, timer=None):
+        self.index = index
+        self.tiles = tuple(tiles)
+        self.find_depths = tuple(find_depths)
+        self.grid_percents = tuple(grid_percents)
+        self.trials = max(1, int(trials))
+        self.timer = timer if timer is not None else _EventTimer()
+        self.table = {}     # (op, length class) -> config dict
+        self.history = {}   # (op, length class) -> [(config, ms per unit), ...]
+
+    # ------------------------------------------------------------------ knobs
+    def apply_config(self, cfg):
+        if "tile" in cfg:
+            self.index.tile_size = cfg["tile"]
+        if "find_depth" in cfg:
+            self.index.set_find_depth(cfg["find_depth"])
+        if "grid_percent" in cfg:
+            self.index.set_multi_grid_percent(cfg["grid_percent"])
+
+    def apply(self, op, max_key_length):
+        """Applies the cached choice for (op, key-length class); False if none."""
+        cfg = self.table.get((op, length_class(max_key_length)))
+        if cfg is None:
+            return False
+        self.apply_config(cfg)
+        return True
+
+    def best(self, op, max_key_length):
+        return self.table.get((op, length_class(max_key_length)))
+
+    # ----------------------------------------------------------------- tuning
+    def _pick(self, key, scored):
+        self.history[key] = scored
+        cfg = min(scored, key=lambda cs: cs[1])[0]
+        self.table[key] = cfg
+        self.apply_config(cfg)
+        return cfg
+
+    def tune_find(self, run_find, max_key_length, warmup=1):
+        """run_find() launches one read-only find batch (the same batch every call).
+        Every (tile, depth) candidate is warmed up, then timed `trials` times; the
+        minimum counts."""
+        scored = []
+        for tile, depth in itertools.product(self.tiles, self.find_depths):
+            cfg = {"tile": tile, "find_depth": depth}
+            self.apply_config(cfg)
+            for _ in range(warmup):
+                self.timer(run_find)
+            ms = min(self.timer(run_find) for _ in range(self.trials))
+            scored.append((cfg, ms))
+        return self._pick(("find", length_class(max_key_length)), scored)
+
+    def tune_online(self, op, run_unit, max_key_length, space=None):
+        """run_unit(cfg) runs the next fresh unit of mutating work (a batch, or a group of
+        batches) under the knobs already applied and returns its size in operations.
+        Each candidate gets `trials` units; the score is milliseconds per operation."""
+        if space is None:
+            space = [{"tile": t, "grid_percent": g}
+                     for t, g in itertools.product(self.tiles, self.grid_percents)]
+        scored = []
+        for cfg in space:
+            self.apply_config(cfg)
+            total_ms, total_ops = 0.0, 0
+            for _ in range(self.trials):
+                box = {}
+                total_ms += self.timer(lambda: box.setdefault("n", run_unit(cfg)))
+                total_ops += int(box["n"])
+            scored.append((dict(cfg), total_ms / max(total_ops, 1)))
+        return self._pick((op, length_class(max_key_length)), scored)
+
+    def report(self):
+        """Rows (op, length class, config, ms per unit or per op) of every measurement."""
+        rows = []
+        for (op, lc), scored in sorted(self.history.items()):
+            for cfg, ms in scored:
+                rows.append((op, lc, cfg, ms))
+        return rows

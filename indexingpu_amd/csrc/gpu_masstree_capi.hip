@@ -1,5 +1,7 @@
 // C ABI over GpuMasstree::gpu_masstree (declared in include/gpu_masstree_c.h).
-// Instantiates the index for tile widths 16, 32 and 64 and dispatches at run time.
+// One index holds the tree for tile widths 16, 32 and 64 at once (the 16-lane instance
+// owns it, the others attach to the same state: the node layout does not depend on the
+// tile width), so the active width is a run-time choice.
 #include <cstring>
 #include <memory>
 #include <string>
@@ -13,100 +15,52 @@ thread_local std::string g_last_error;
 
 using alloc_t = indexingpu_amd::simple_slab_allocator<128>;
 using reclaim_t = indexingpu_amd::simple_debra_reclaimer<>;
-
-struct index_base {
-  virtual ~index_base() = default;
-  virtual uint32_t tile() const = 0;
-  virtual void find(const uint32_t*, uint32_t, const uint32_t*, uint32_t*, uint32_t, uint32_t*,
-                    uint32_t, bool, hipStream_t) = 0;
-  virtual void insert(const uint32_t*, uint32_t, const uint32_t*, const uint32_t*, uint32_t,
-                      const uint32_t*, uint32_t, bool, bool*, hipStream_t) = 0;
-  virtual void update(const uint32_t*, uint32_t, const uint32_t*, const uint32_t*, uint32_t,
-                      const uint32_t*, uint32_t, bool*, hipStream_t) = 0;
-  virtual void erase(const uint32_t*, uint32_t, const uint32_t*, uint32_t, bool*, hipStream_t) = 0;
-  virtual void scan(const uint32_t*, const uint32_t*, uint32_t, uint32_t, uint32_t, const uint32_t*,
-                    const uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t*, uint32_t*,
-                    uint32_t*, const uint32_t*, hipStream_t) = 0;
-  virtual void mixed(const uint8_t*, const uint32_t*, uint32_t, const uint32_t*, uint32_t*,
-                     uint32_t, uint32_t*, bool*, uint32_t, hipStream_t) = 0;
-  virtual uint32_t device_error(bool clear) = 0;
-  virtual indexingpu_amd::walk_stats walk(const indexingpu_amd::entry_emit_fn&) = 0;
-  virtual uint64_t slabs_allocated() = 0;
-  virtual uint64_t pool_slabs() = 0;
-  virtual void set_find_depth(uint32_t k) = 0;
-};
-
 template <uint32_t T>
-struct index_impl final : index_base {
+using tree_t = GpuMasstree::gpu_masstree<alloc_t, reclaim_t, T>;
+
+struct gmt_impl {
   std::unique_ptr<alloc_t> alloc;
   std::unique_ptr<reclaim_t> reclaim;
-  std::unique_ptr<GpuMasstree::gpu_masstree<alloc_t, reclaim_t, T>> tree;
+  std::unique_ptr<tree_t<16>> t16;
+  std::unique_ptr<tree_t<32>> t32;
+  std::unique_ptr<tree_t<64>> t64;
+  uint32_t tile = 16;
 
-  index_impl(uint64_t pool_bytes, float pool_ratio, uint32_t retire_capacity) {
+  gmt_impl(uint32_t tile_size, uint64_t pool_bytes, float pool_ratio, uint32_t retire_capacity)
+      : tile(tile_size) {
     if (pool_bytes)
       alloc.reset(new alloc_t(indexingpu_amd::pool_bytes{pool_bytes}));
     else
       alloc.reset(new alloc_t(pool_ratio > 0.f ? pool_ratio : 0.9f));
     reclaim.reset(retire_capacity ? new reclaim_t(retire_capacity) : new reclaim_t());
-    tree.reset(new GpuMasstree::gpu_masstree<alloc_t, reclaim_t, T>(*alloc, *reclaim));
+    t16.reset(new tree_t<16>(*alloc, *reclaim));
+    t32.reset(new tree_t<32>(*t16));
+    t64.reset(new tree_t<64>(*t16));
   }
-  ~index_impl() override {
+  ~gmt_impl() {
     (void)hipDeviceSynchronize();
-    tree.reset();
+    t64.reset();
+    t32.reset();
+    t16.reset();
     reclaim.reset();
     alloc.reset();
   }
-  uint32_t tile() const override { return T; }
-  void find(const uint32_t* k, uint32_t mk, const uint32_t* kl, uint32_t* v, uint32_t mv,
-            uint32_t* vl, uint32_t n, bool conc, hipStream_t s) override {
-    if (conc)
-      tree->template find<true>(k, mk, kl, v, mv, vl, n, s);
-    else
-      tree->template find<false>(k, mk, kl, v, mv, vl, n, s);
+  // runs f on the instance of the active tile width
+  template <class F>
+  void with_tree(F&& f) {
+    switch (tile) {
+      case 32: f(*t32); break;
+      case 64: f(*t64); break;
+      default: f(*t16); break;
+    }
   }
-  void insert(const uint32_t* k, uint32_t mk, const uint32_t* kl, const uint32_t* v, uint32_t mv,
-              const uint32_t* vl, uint32_t n, bool upsert, bool* r, hipStream_t s) override {
-    if (upsert)
-      tree->template insert<true>(k, mk, kl, v, mv, vl, n, s, r);
-    else
-      tree->template insert<false>(k, mk, kl, v, mv, vl, n, s, r);
+  // settings that live per instance are applied to all three
+  template <class F>
+  void on_all(F&& f) {
+    f(*t16);
+    f(*t32);
+    f(*t64);
   }
-  void update(const uint32_t* k, uint32_t mk, const uint32_t* kl, const uint32_t* v, uint32_t mv,
-              const uint32_t* vl, uint32_t n, bool* r, hipStream_t s) override {
-    tree->update(k, mk, kl, v, mv, vl, n, s, r);
-  }
-  void erase(const uint32_t* k, uint32_t mk, const uint32_t* kl, uint32_t n, bool* r,
-             hipStream_t s) override {
-    tree->erase(k, mk, kl, n, s, r);
-  }
-  void scan(const uint32_t* lk, const uint32_t* ll, uint32_t mk, uint32_t mc, uint32_t nq,
-            const uint32_t* uk, const uint32_t* ul, uint32_t* counts, uint32_t* vals, uint32_t mv,
-            uint32_t* vlens, uint32_t* ok, uint32_t* okl, const uint32_t* lim,
-            hipStream_t s) override {
-    if (uk)
-      tree->template scan<true>(lk, ll, mk, mc, nq, uk, ul, counts, vals, mv, vlens, ok, okl, s, lim);
-    else
-      tree->template scan<false>(lk, ll, mk, mc, nq, nullptr, nullptr, counts, vals, mv, vlens, ok,
-                                 okl, s, lim);
-  }
-  void mixed(const uint8_t* t, const uint32_t* k, uint32_t mk, const uint32_t* kl, uint32_t* v,
-             uint32_t mv, uint32_t* vl, bool* r, uint32_t n, hipStream_t s) override {
-    tree->mixed_batch(reinterpret_cast<const kernels::request_type*>(t), k, mk, kl, v, mv, vl, r,
-                      n, s);
-  }
-  uint32_t device_error(bool clear) override {
-    IGPU_HIP_TRY(hipDeviceSynchronize());
-    uint32_t h = 0;
-    IGPU_HIP_TRY(hipMemcpy(&h, tree->err_, 4, hipMemcpyDeviceToHost));
-    if (clear) IGPU_HIP_TRY(hipMemset(tree->err_, 0, 4));
-    return h;
-  }
-  indexingpu_amd::walk_stats walk(const indexingpu_amd::entry_emit_fn& f) override {
-    return tree->walk(f);
-  }
-  uint64_t slabs_allocated() override { return alloc->num_allocated_slabs(); }
-  uint64_t pool_slabs() override { return alloc->total_slabs(); }
-  void set_find_depth(uint32_t k) override { tree->set_find_depth(k); }
 };
 
 template <class F>
@@ -125,13 +79,15 @@ int guarded(F&& f) {
   }
 }
 
-inline index_base* B(gmt_index* h) { return reinterpret_cast<index_base*>(h); }
+inline gmt_impl* B(gmt_index* h) { return reinterpret_cast<gmt_impl*>(h); }
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 int bad(const char* msg) {
   g_last_error = msg;
   return GMT_EINVAL;
 }
+
+bool valid_tile(uint32_t t) { return t == 16 || t == 32 || t == 64; }
 
 }  // namespace
 
@@ -141,17 +97,10 @@ int gmt_create(uint32_t tile_size, uint64_t pool_bytes, float pool_ratio,
                uint32_t retire_capacity, gmt_index** out) {
   if (!out) return bad("gmt_create: null output handle");
   *out = nullptr;
-  index_base* p = nullptr;
-  int rc = guarded([&] {
-    switch (tile_size) {
-      case 16: p = new index_impl<16>(pool_bytes, pool_ratio, retire_capacity); break;
-      case 32: p = new index_impl<32>(pool_bytes, pool_ratio, retire_capacity); break;
-      case 64: p = new index_impl<64>(pool_bytes, pool_ratio, retire_capacity); break;
-      default: break;
-    }
-  });
+  if (!valid_tile(tile_size)) return bad("gmt_create: tile_size must be 16, 32 or 64");
+  gmt_impl* p = nullptr;
+  int rc = guarded([&] { p = new gmt_impl(tile_size, pool_bytes, pool_ratio, retire_capacity); });
   if (rc != GMT_OK) return rc;
-  if (!p) return bad("gmt_create: tile_size must be 16, 32 or 64");
   *out = reinterpret_cast<gmt_index*>(p);
   return GMT_OK;
 }
@@ -168,8 +117,14 @@ int gmt_find(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
   if (num_keys && (!keys || !value_lengths)) return bad("gmt_find: null keys or value_lengths");
   if (num_keys && max_value_length && !values) return bad("gmt_find: null values");
   return guarded([&] {
-    B(index)->find(keys, max_key_length, key_lengths, values, max_value_length, value_lengths,
-                   num_keys, concurrent != 0, S(stream));
+    B(index)->with_tree([&](auto& t) {
+      if (concurrent)
+        t.template find<true>(keys, max_key_length, key_lengths, values, max_value_length,
+                              value_lengths, num_keys, S(stream));
+      else
+        t.template find<false>(keys, max_key_length, key_lengths, values, max_value_length,
+                               value_lengths, num_keys, S(stream));
+    });
   });
 }
 
@@ -180,9 +135,16 @@ int gmt_insert(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
   if (!index) return bad("gmt_insert: null handle");
   if (num_keys && (!keys || (!values && max_value_length)))
     return bad("gmt_insert: null keys or values");
+  bool* r = reinterpret_cast<bool*>(results);
   return guarded([&] {
-    B(index)->insert(keys, max_key_length, key_lengths, values, max_value_length, value_lengths,
-                     num_keys, update_if_exists != 0, reinterpret_cast<bool*>(results), S(stream));
+    B(index)->with_tree([&](auto& t) {
+      if (update_if_exists)
+        t.template insert<true>(keys, max_key_length, key_lengths, values, max_value_length,
+                                value_lengths, num_keys, S(stream), r);
+      else
+        t.template insert<false>(keys, max_key_length, key_lengths, values, max_value_length,
+                                 value_lengths, num_keys, S(stream), r);
+    });
   });
 }
 
@@ -192,9 +154,12 @@ int gmt_update(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
   if (!index) return bad("gmt_update: null handle");
   if (num_keys && (!keys || (!values && max_value_length)))
     return bad("gmt_update: null keys or values");
+  bool* r = reinterpret_cast<bool*>(results);
   return guarded([&] {
-    B(index)->update(keys, max_key_length, key_lengths, values, max_value_length, value_lengths,
-                     num_keys, reinterpret_cast<bool*>(results), S(stream));
+    B(index)->with_tree([&](auto& t) {
+      t.update(keys, max_key_length, key_lengths, values, max_value_length, value_lengths,
+               num_keys, S(stream), r);
+    });
   });
 }
 
@@ -202,9 +167,10 @@ int gmt_erase(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
               const uint32_t* key_lengths, uint32_t num_keys, uint8_t* results, void* stream) {
   if (!index) return bad("gmt_erase: null handle");
   if (num_keys && !keys) return bad("gmt_erase: null keys");
+  bool* r = reinterpret_cast<bool*>(results);
   return guarded([&] {
-    B(index)->erase(keys, max_key_length, key_lengths, num_keys, reinterpret_cast<bool*>(results),
-                    S(stream));
+    B(index)->with_tree(
+        [&](auto& t) { t.erase(keys, max_key_length, key_lengths, num_keys, S(stream), r); });
   });
 }
 
@@ -217,9 +183,18 @@ int gmt_scan(gmt_index* index, const uint32_t* lower_keys, const uint32_t* lower
   if (!index) return bad("gmt_scan: null handle");
   if (num_queries && !lower_keys) return bad("gmt_scan: null lower_keys");
   return guarded([&] {
-    B(index)->scan(lower_keys, lower_key_lengths, max_key_length, max_count_per_query, num_queries,
-                   upper_keys, upper_key_lengths, counts, values, max_value_length, value_lengths,
-                   out_keys, out_key_lengths, per_query_limits, S(stream));
+    B(index)->with_tree([&](auto& t) {
+      if (upper_keys)
+        t.template scan<true>(lower_keys, lower_key_lengths, max_key_length, max_count_per_query,
+                              num_queries, upper_keys, upper_key_lengths, counts, values,
+                              max_value_length, value_lengths, out_keys, out_key_lengths,
+                              S(stream), per_query_limits);
+      else
+        t.template scan<false>(lower_keys, lower_key_lengths, max_key_length,
+                               max_count_per_query, num_queries, nullptr, nullptr, counts, values,
+                               max_value_length, value_lengths, out_keys, out_key_lengths,
+                               S(stream), per_query_limits);
+    });
   });
 }
 
@@ -231,23 +206,49 @@ int gmt_mixed_batch(gmt_index* index, const uint8_t* types, const uint32_t* keys
   if (num_requests && (!types || !keys || !value_lengths))
     return bad("gmt_mixed_batch: null types, keys or value_lengths");
   return guarded([&] {
-    B(index)->mixed(types, keys, max_key_length, key_lengths, values, max_value_length,
-                    value_lengths, reinterpret_cast<bool*>(results), num_requests, S(stream));
+    B(index)->with_tree([&](auto& t) {
+      t.mixed_batch(reinterpret_cast<const kernels::request_type*>(types), keys, max_key_length,
+                    key_lengths, values, max_value_length, value_lengths,
+                    reinterpret_cast<bool*>(results), num_requests, S(stream));
+    });
+  });
+}
+
+int gmt_multi_batch(gmt_index* index, const uint8_t* types, const uint32_t* keys,
+                    uint32_t max_key_length, const uint32_t* key_lengths, uint32_t* values,
+                    uint32_t max_value_length, uint32_t* value_lengths, uint8_t* results,
+                    const uint64_t* offsets, uint32_t num_batches, uint64_t total_requests,
+                    void* stream) {
+  if (!index) return bad("gmt_multi_batch: null handle");
+  if (num_batches && (!types || !keys || !value_lengths || !offsets))
+    return bad("gmt_multi_batch: null types, keys, value_lengths or offsets");
+  return guarded([&] {
+    B(index)->with_tree([&](auto& t) {
+      t.multi_batch(reinterpret_cast<const kernels::request_type*>(types), keys, max_key_length,
+                    key_lengths, values, max_value_length, value_lengths,
+                    reinterpret_cast<bool*>(results), offsets, num_batches, total_requests,
+                    S(stream));
+    });
   });
 }
 
 uint32_t gmt_device_error(gmt_index* index, int clear) {
   if (!index) return 0xFFFFFFFFu;
   uint32_t e = 0xFFFFFFFFu;
-  if (guarded([&] { e = B(index)->device_error(clear != 0); }) != GMT_OK) return 0xFFFFFFFFu;
-  return e;
+  int rc = guarded([&] {
+    uint32_t* d = B(index)->t16->err_;
+    IGPU_HIP_TRY(hipDeviceSynchronize());
+    IGPU_HIP_TRY(hipMemcpy(&e, d, 4, hipMemcpyDeviceToHost));
+    if (clear) IGPU_HIP_TRY(hipMemset(d, 0, 4));
+  });
+  return rc == GMT_OK ? e : 0xFFFFFFFFu;
 }
 
 int gmt_validate(gmt_index* index, uint64_t* entries, uint64_t* nodes, uint64_t* record_slabs,
                  char* msg, uint32_t msg_len) {
   if (!index) return bad("gmt_validate: null handle");
   indexingpu_amd::walk_stats s;
-  int rc = guarded([&] { s = B(index)->walk(nullptr); });
+  int rc = guarded([&] { s = B(index)->t16->walk(nullptr); });
   if (rc != GMT_OK) return -rc;
   if (entries) *entries = s.entries;
   if (nodes) *nodes = s.nodes;
@@ -266,7 +267,7 @@ int64_t gmt_dump(gmt_index* index, uint32_t* key_offsets, uint32_t* key_data,
   uint64_t ks = 0, vs = 0, i = 0;
   indexingpu_amd::walk_stats s;
   int rc = guarded([&] {
-    s = B(index)->walk([&](const std::vector<uint32_t>& k, const std::vector<uint32_t>& v) {
+    s = B(index)->t16->walk([&](const std::vector<uint32_t>& k, const std::vector<uint32_t>& v) {
       if (key_offsets) key_offsets[i] = (uint32_t)ks;
       if (value_offsets) value_offsets[i] = (uint32_t)vs;
       if (key_data) std::memcpy(key_data + ks, k.data(), k.size() * 4);
@@ -291,19 +292,32 @@ int64_t gmt_dump(gmt_index* index, uint32_t* key_offsets, uint32_t* key_data,
 uint64_t gmt_slabs_allocated(gmt_index* index) {
   if (!index) return 0;
   uint64_t v = 0;
-  guarded([&] { v = B(index)->slabs_allocated(); });
+  guarded([&] { v = B(index)->alloc->num_allocated_slabs(); });
   return v;
 }
 
-uint64_t gmt_pool_slabs(gmt_index* index) { return index ? B(index)->pool_slabs() : 0; }
+uint64_t gmt_pool_slabs(gmt_index* index) { return index ? B(index)->alloc->total_slabs() : 0; }
 
-int gmt_set_find_depth(gmt_index* index, uint32_t depth) {
-  if (!index) return bad("gmt_set_find_depth: null handle");
-  B(index)->set_find_depth(depth);
+int gmt_set_multi_grid_percent(gmt_index* index, uint32_t percent) {
+  if (!index) return bad("gmt_set_multi_grid_percent: null handle");
+  B(index)->on_all([&](auto& t) { t.set_multi_grid_percent(percent); });
   return GMT_OK;
 }
 
-uint32_t gmt_tile_size(gmt_index* index) { return index ? B(index)->tile() : 0; }
+int gmt_set_find_depth(gmt_index* index, uint32_t depth) {
+  if (!index) return bad("gmt_set_find_depth: null handle");
+  B(index)->on_all([&](auto& t) { t.set_find_depth(depth); });
+  return GMT_OK;
+}
+
+int gmt_set_tile_size(gmt_index* index, uint32_t tile_size) {
+  if (!index) return bad("gmt_set_tile_size: null handle");
+  if (!valid_tile(tile_size)) return bad("gmt_set_tile_size: tile_size must be 16, 32 or 64");
+  B(index)->tile = tile_size;
+  return GMT_OK;
+}
+
+uint32_t gmt_tile_size(gmt_index* index) { return index ? B(index)->tile : 0; }
 
 const char* gmt_last_error_message(void) { return g_last_error.c_str(); }
 

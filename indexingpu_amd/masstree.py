@@ -68,6 +68,9 @@ def load_library():
     lib.gmt_pool_slabs.argtypes = [vp]
     lib.gmt_pool_slabs.restype = u64
     lib.gmt_set_find_depth.argtypes = [vp, u32]
+    lib.gmt_multi_batch.argtypes = [vp, P, P, u32, P, P, u32, P, P, P, u32, u64, vp]
+    lib.gmt_set_multi_grid_percent.argtypes = [vp, u32]
+    lib.gmt_set_tile_size.argtypes = [vp, u32]
     lib.gmt_tile_size.argtypes = [vp]
     lib.gmt_tile_size.restype = u32
     lib.gmt_last_error_message.restype = ctypes.c_char_p
@@ -75,8 +78,9 @@ def load_library():
     return lib
 
 
-def _ptr(t, esize, name):
-    """Device pointer of a contiguous CUDA tensor with elements of `esize` bytes."""
+def _ptr(t, esize, name, need=0):
+    """Device pointer of a contiguous CUDA tensor with elements of `esize` bytes and at
+    least `need` elements (checked here, so a kernel never indexes past a buffer)."""
     if t is None:
         return None
     if not isinstance(t, torch.Tensor):
@@ -87,6 +91,8 @@ def _ptr(t, esize, name):
         raise ValueError(f"{name} must be contiguous")
     if not t.is_cuda:
         raise ValueError(f"{name} must be a GPU tensor")
+    if t.numel() < need:
+        raise ValueError(f"{name} holds {t.numel()} elements, the call needs {need}")
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -111,7 +117,6 @@ class GpuMasstree:
         self._check(self._lib.gmt_create(tile_size, int(pool_bytes), float(pool_ratio),
                                          int(retire_capacity), ctypes.byref(h)))
         self._h = h
-        self.tile_size = tile_size
 
     def close(self):
         if getattr(self, "_h", None):
@@ -132,50 +137,67 @@ class GpuMasstree:
     # ------------------------------------------------------------------ batch API
     def find(self, keys, max_key_length, key_lengths, values, max_value_length, value_lengths,
              num_keys, concurrent=False, stream=None):
+        n = int(num_keys)
         self._check(self._lib.gmt_find(
-            self._h, _ptr(keys, 4, "keys"), max_key_length, _ptr(key_lengths, 4, "key_lengths"),
-            _ptr(values, 4, "values"), max_value_length, _ptr(value_lengths, 4, "value_lengths"),
-            num_keys, int(bool(concurrent)), _stream(stream)))
+            self._h, _ptr(keys, 4, "keys", n * max_key_length), max_key_length,
+            _ptr(key_lengths, 4, "key_lengths", n),
+            _ptr(values, 4, "values", n * max_value_length), max_value_length,
+            _ptr(value_lengths, 4, "value_lengths", n), n, int(bool(concurrent)),
+            _stream(stream)))
 
     def insert(self, keys, max_key_length, key_lengths, values, max_value_length, value_lengths,
                num_keys, update_if_exists=False, results=None, stream=None):
+        n = int(num_keys)
         self._check(self._lib.gmt_insert(
-            self._h, _ptr(keys, 4, "keys"), max_key_length, _ptr(key_lengths, 4, "key_lengths"),
-            _ptr(values, 4, "values"), max_value_length, _ptr(value_lengths, 4, "value_lengths"),
-            num_keys, int(bool(update_if_exists)), _ptr(results, 1, "results"), _stream(stream)))
+            self._h, _ptr(keys, 4, "keys", n * max_key_length), max_key_length,
+            _ptr(key_lengths, 4, "key_lengths", n),
+            _ptr(values, 4, "values", n * max_value_length), max_value_length,
+            _ptr(value_lengths, 4, "value_lengths", n), n, int(bool(update_if_exists)),
+            _ptr(results, 1, "results", n), _stream(stream)))
 
     def update(self, keys, max_key_length, key_lengths, values, max_value_length, value_lengths,
                num_keys, results=None, stream=None):
+        n = int(num_keys)
         self._check(self._lib.gmt_update(
-            self._h, _ptr(keys, 4, "keys"), max_key_length, _ptr(key_lengths, 4, "key_lengths"),
-            _ptr(values, 4, "values"), max_value_length, _ptr(value_lengths, 4, "value_lengths"),
-            num_keys, _ptr(results, 1, "results"), _stream(stream)))
+            self._h, _ptr(keys, 4, "keys", n * max_key_length), max_key_length,
+            _ptr(key_lengths, 4, "key_lengths", n),
+            _ptr(values, 4, "values", n * max_value_length), max_value_length,
+            _ptr(value_lengths, 4, "value_lengths", n), n, _ptr(results, 1, "results", n),
+            _stream(stream)))
 
     def erase(self, keys, max_key_length, key_lengths, num_keys, results=None, stream=None):
+        n = int(num_keys)
         self._check(self._lib.gmt_erase(
-            self._h, _ptr(keys, 4, "keys"), max_key_length, _ptr(key_lengths, 4, "key_lengths"),
-            num_keys, _ptr(results, 1, "results"), _stream(stream)))
+            self._h, _ptr(keys, 4, "keys", n * max_key_length), max_key_length,
+            _ptr(key_lengths, 4, "key_lengths", n), n, _ptr(results, 1, "results", n),
+            _stream(stream)))
 
     def scan(self, lower_keys, lower_key_lengths, max_key_length, max_count_per_query,
              num_queries, upper_keys=None, upper_key_lengths=None, counts=None, values=None,
              max_value_length=1, value_lengths=None, out_keys=None, out_key_lengths=None,
              per_query_limits=None, stream=None):
+        q, c = int(num_queries), int(max_count_per_query)
         self._check(self._lib.gmt_scan(
-            self._h, _ptr(lower_keys, 4, "lower_keys"),
-            _ptr(lower_key_lengths, 4, "lower_key_lengths"), max_key_length,
-            max_count_per_query, num_queries, _ptr(upper_keys, 4, "upper_keys"),
-            _ptr(upper_key_lengths, 4, "upper_key_lengths"), _ptr(counts, 4, "counts"),
-            _ptr(values, 4, "values"), max_value_length, _ptr(value_lengths, 4, "value_lengths"),
-            _ptr(out_keys, 4, "out_keys"), _ptr(out_key_lengths, 4, "out_key_lengths"),
-            _ptr(per_query_limits, 4, "per_query_limits"), _stream(stream)))
+            self._h, _ptr(lower_keys, 4, "lower_keys", q * max_key_length),
+            _ptr(lower_key_lengths, 4, "lower_key_lengths", q), max_key_length, c, q,
+            _ptr(upper_keys, 4, "upper_keys", q * max_key_length),
+            _ptr(upper_key_lengths, 4, "upper_key_lengths", q), _ptr(counts, 4, "counts", q),
+            _ptr(values, 4, "values", q * c * max_value_length), max_value_length,
+            _ptr(value_lengths, 4, "value_lengths", q * c),
+            _ptr(out_keys, 4, "out_keys", q * c * max_key_length),
+            _ptr(out_key_lengths, 4, "out_key_lengths", q * c),
+            _ptr(per_query_limits, 4, "per_query_limits", q), _stream(stream)))
 
     def mixed_batch(self, request_types, keys, max_key_length, key_lengths, values,
                     max_value_length, value_lengths, results, num_requests, stream=None):
+        n = int(num_requests)
         self._check(self._lib.gmt_mixed_batch(
-            self._h, _ptr(request_types, 1, "request_types"), _ptr(keys, 4, "keys"),
-            max_key_length, _ptr(key_lengths, 4, "key_lengths"), _ptr(values, 4, "values"),
-            max_value_length, _ptr(value_lengths, 4, "value_lengths"),
-            _ptr(results, 1, "results"), num_requests, _stream(stream)))
+            self._h, _ptr(request_types, 1, "request_types", n),
+            _ptr(keys, 4, "keys", n * max_key_length), max_key_length,
+            _ptr(key_lengths, 4, "key_lengths", n),
+            _ptr(values, 4, "values", n * max_value_length), max_value_length,
+            _ptr(value_lengths, 4, "value_lengths", n), _ptr(results, 1, "results", n), n,
+            _stream(stream)))
 
     # ------------------------------------------------------------------ checks
     def device_error(self, clear=True):
@@ -213,6 +235,33 @@ class GpuMasstree:
         if n2 != n:
             raise MasstreeError("tree changed between dump calls")
         return ko, kd[:ks.value], vo, vd[:vs.value]
+
+    def multi_batch(self, request_types, keys, max_key_length, key_lengths, values,
+                    max_value_length, value_lengths, results, offsets, num_batches,
+                    total_requests, stream=None):
+        """Batched form: applies num_batches mixed batches (stored back to back, split by
+        the int64 `offsets` tensor of num_batches + 1 entries) in order, in one launch."""
+        n = int(total_requests)
+        self._check(self._lib.gmt_multi_batch(
+            self._h, _ptr(request_types, 1, "request_types", n),
+            _ptr(keys, 4, "keys", n * max_key_length), max_key_length,
+            _ptr(key_lengths, 4, "key_lengths", n),
+            _ptr(values, 4, "values", n * max_value_length), max_value_length,
+            _ptr(value_lengths, 4, "value_lengths", n), _ptr(results, 1, "results", n),
+            _ptr(offsets, 8, "offsets", int(num_batches) + 1), num_batches, n,
+            _stream(stream)))
+
+    def set_multi_grid_percent(self, p):
+        self._check(self._lib.gmt_set_multi_grid_percent(self._h, int(p)))
+
+    @property
+    def tile_size(self):
+        """Lanes per operation (16, 32 or 64); settable between calls."""
+        return int(self._lib.gmt_tile_size(self._h))
+
+    @tile_size.setter
+    def tile_size(self, t):
+        self._check(self._lib.gmt_set_tile_size(self._h, int(t)))
 
     def set_find_depth(self, k):
         """Lookups each tile keeps in flight in read-only find batches (1, 2, 4, 8)."""

@@ -198,31 +198,45 @@ class MixedWorkload:
         self.live = np.ones(n_preload, bool)
         self.hashes = key_hash64(self.keys, self.lens)
 
-    def batch(self, n_ops):
+    def batch(self, n_ops, check_fresh=True):
+        """One batch: types, keys, lens, values (input for inserts), plus the expected
+        find result of every request (`expect_len`, 0 = absent) from the live-set model.
+        check_fresh=False skips the membership test of fresh keys against every key ever
+        generated (for keys of >= 2 random slices a collision has probability < 2^-60)."""
         rng = self.rng
         nf = int(n_ops * self.find_frac)
         ni = int(n_ops * self.insert_frac)
         ne = n_ops - nf - ni
         ranks = self.zipf.sample(rng, nf)
         fidx = feistel_permute(ranks, self.n_preload, seed=0).astype(np.int64)
-        read = np.zeros(self.n_preload, bool)
-        read[fidx] = True
-        cand = np.flatnonzero(self.live & ~read)
-        eidx = rng.choice(cand, size=min(ne, len(cand)), replace=False)
+        cand = rng.integers(0, self.n_preload, size=4 * ne + 64)
+        cand = cand[self.live[cand] & ~np.isin(cand, fidx)]
+        _, first = np.unique(cand, return_index=True)
+        eidx = cand[np.sort(first)][:ne]
         ne = len(eidx)
+        find_live = self.live[fidx].copy()
         self.live[eidx] = False
         ik, il = unique_random_keys(rng, ni, self.key_len, self.key_len,
-                                    exclude_hashes=self.hashes)
-        self.hashes = np.concatenate([self.hashes, key_hash64(ik, il)])
+                                    exclude_hashes=self.hashes if check_fresh else None)
+        if check_fresh:
+            self.hashes = np.concatenate([self.hashes, key_hash64(ik, il)])
         iv = make_values(ik, il, self.max_value_length)
         types = np.concatenate([np.full(nf, 2), np.full(ni, 0), np.full(ne, 1)]).astype(np.uint8)
         keys = np.concatenate([self.keys[fidx], ik, self.keys[eidx]])
         lens = np.concatenate([self.lens[fidx], il, self.lens[eidx]])
         vals = np.concatenate([np.zeros((nf, self.max_value_length), U32), iv,
                                np.zeros((ne, self.max_value_length), U32)])
+        # expected outcome: finds hit iff the key is still live; inserts of fresh keys and
+        # erases of live keys succeed
+        expect_len = np.concatenate([np.where(find_live, self.max_value_length, 0),
+                                     np.zeros(ni + ne, np.int64)]).astype(U32)
+        expect_val = np.concatenate([np.where(find_live[:, None], self.values[fidx], 0),
+                                     iv, np.zeros((ne, self.max_value_length), U32)]).astype(U32)
         perm = rng.permutation(len(types))
         return dict(types=types[perm].copy(), keys=np.ascontiguousarray(keys[perm]),
-                    lens=lens[perm].copy(), values=np.ascontiguousarray(vals[perm]))
+                    lens=lens[perm].copy(), values=np.ascontiguousarray(vals[perm]),
+                    expect_len=expect_len[perm].copy(),
+                    expect_values=np.ascontiguousarray(expect_val[perm]))
 
     def scans(self, n_queries, max_scan=100):
         rng = self.rng

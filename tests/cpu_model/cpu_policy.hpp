@@ -60,7 +60,8 @@ struct cpu_tile {
 
   // ---------------- node reads / locking ----------------
   template <bool Validate>
-  bool read_node(u32 idx, node_t& n) {
+  bool read_node(u32 idx, node_t& n, bool cached_ok = false) {
+    (void)cached_ok;  // host caches are coherent: every read is the validated one
     if (!check(idx)) return false;
     const u64* p = P.slab(idx);
     if (!Validate) {
@@ -107,7 +108,13 @@ struct cpu_tile {
   }
   void write_node(u32 idx, const node_t& n) {
     u64* p = P.slab(idx);
-    for (u32 i = 0; i < kSlots; i++) st(p + i, n.s[i]);
+    const u32 h = header(n);
+    for (u32 i = 0; i < kSlots; i++) {
+      u64 s = n.s[i];
+      if (!hdr_border(h) && i < kTagSlots)
+        s = (s & ~(1ull << 63)) | ((u64)((hdr_publish_tag(h) >> i) & 1u) << 63);
+      st(p + i, s);
+    }
   }
   void fence_release() { __atomic_thread_fence(__ATOMIC_RELEASE); }
 
@@ -125,13 +132,13 @@ struct cpu_tile {
   u32 find_child(const node_t& n, u32 q) const {
     u32 nk = nkeys_of(n);
     for (u32 i = 0; i < nk; i++)
-      if (q <= slot_lo(n.s[i])) return slot_hi(n.s[i]);
+      if (q <= slot_lo(n.s[i])) return child_of(n.s[i]);
     return kInvalid;
   }
   bool find_child_slot(const node_t& n, u32 child, u32& j) const {
     u32 nk = nkeys_of(n);
     for (u32 i = 0; i < nk; i++)
-      if (slot_hi(n.s[i]) == child) {
+      if (child_of(n.s[i]) == child) {
         j = i;
         return true;
       }

@@ -1,0 +1,83 @@
+"""Host logic of the autotuner: candidate space, scoring, caching per key-length class
+and knob application, with a fake index and a fake clock (no GPU)."""
+from indexingpu_amd.autotune import Autotuner, length_class
+
+
+class FakeIndex:
+    def __init__(self):
+        self.tile_size = 16
+        self.depth = 4
+        self.grid = 100
+        self.log = []
+
+    def set_find_depth(self, k):
+        self.depth = k
+
+    def set_multi_grid_percent(self, p):
+        self.grid = p
+
+
+class FakeClock:
+    """Milliseconds = cost(knobs of the index at the time fn runs)."""
+
+    def __init__(self, index, cost):
+        self.index, self.cost = index, cost
+        self.calls = 0
+
+    def __call__(self, fn):
+        fn()
+        self.calls += 1
+        return self.cost(self.index.tile_size, self.index.depth, self.index.grid)
+
+
+def test_length_class():
+    assert [length_class(m) for m in (0, 1, 2, 3, 4, 5, 8, 9, 16, 17)] == \
+        [1, 1, 2, 4, 4, 8, 8, 16, 16, 32]
+
+
+def test_tune_find_picks_minimum_and_applies_it():
+    idx = FakeIndex()
+    cost = lambda t, d, g: abs(t - 32) + abs(d - 2) * 3 + 1.0
+    clock = FakeClock(idx, cost)
+    tu = Autotuner(idx, trials=2, timer=clock)
+    runs = []
+    cfg = tu.tune_find(lambda: runs.append((idx.tile_size, idx.depth)), max_key_length=8)
+    assert cfg == {"tile": 32, "find_depth": 2}
+    assert (idx.tile_size, idx.depth) == (32, 2)
+    # 3 tiles x 4 depths, each 1 warmup + 2 timed runs
+    assert len(runs) == 12 * 3 and clock.calls == 12 * 3
+    assert tu.best("find", 7) == cfg          # same length class (8)
+    assert tu.best("find", 9) is None          # class 16 not tuned
+    assert len(tu.report()) == 12
+
+
+def test_tune_online_scores_per_operation_and_never_reuses_units():
+    idx = FakeIndex()
+    # time per unit grows with unit size; per-op cost best at tile 64 / grid 50
+    sizes = iter(range(1000, 100000, 37))
+    served = []
+
+    def run_unit(cfg):
+        n = next(sizes)
+        served.append(n)
+        return n
+
+    def cost(t, d, g):
+        return served[-1] * (1.0 + abs(t - 64) / 64 + abs(g - 50) / 100)
+
+    tu = Autotuner(idx, trials=2, timer=FakeClock(idx, cost))
+    cfg = tu.tune_online("multi", run_unit, max_key_length=16)
+    assert cfg == {"tile": 64, "grid_percent": 50}
+    assert len(served) == len(set(served)) == 9 * 2
+    assert (idx.tile_size, idx.grid) == (64, 50)
+
+
+def test_apply_restores_cached_choice():
+    idx = FakeIndex()
+    tu = Autotuner(idx, timer=FakeClock(idx, lambda t, d, g: t + d))
+    tu.tune_find(lambda: None, max_key_length=4)
+    assert (idx.tile_size, idx.depth) == (16, 1)
+    idx.tile_size, idx.depth = 64, 8
+    assert tu.apply("find", 3)
+    assert (idx.tile_size, idx.depth) == (16, 1)
+    assert not tu.apply("mixed", 3)

@@ -1,0 +1,113 @@
+"""Batched form (gmt_multi_batch): many uneven mixed batches applied in order by one
+persistent launch.  Parity: every per-request output equals the oracle replaying the
+batches one after another; batches that touch the same keys in turn check the order."""
+import numpy as np
+import pytest
+import torch
+
+import scenarios as S
+from indexingpu_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _concat(batches, mk, mv):
+    types = np.concatenate([b["types"] for b in batches])
+    keys = np.concatenate([b["keys"] for b in batches])
+    lens = np.concatenate([b["lens"] for b in batches])
+    vals = np.concatenate([b["values"] for b in batches])
+    offs = np.cumsum([0] + [len(b["types"]) for b in batches]).astype(np.int64)
+    return types, keys, lens, vals, offs
+
+
+def _run_multi(g, types, keys, lens, vals, offs, mk, mv):
+    n = len(types)
+    v = g._d(vals)
+    vl = g._d(np.full(n, mv, np.uint32))
+    r = g._zeros(n, np.uint8)
+    off = torch.from_numpy(offs).cuda()
+    g.t.multi_batch(g._d(types, np.uint8), g._d(keys), mk, g._d(lens), v, mv, vl, r, off,
+                    len(offs) - 1, n)
+    out = (g._h(v)[:n * mv].reshape(n, mv), g._h(vl)[:n], g._h(r, np.uint8)[:n])
+    g.t.check_device_error()
+    return out
+
+
+@pytest.mark.parametrize("tile", [16, 64])
+def test_uneven_batches_match_sequential_oracle(tile):
+    wl = W.MixedWorkload(50_000, key_len=4, seed=31)
+    mk, mv = 4, 2
+    rng = np.random.default_rng(5)
+    sizes = np.exp(rng.uniform(np.log(16), np.log(20_000), size=40)).astype(int)
+    batches = [wl.batch(int(s)) for s in sizes]
+    g, o = S.GpuBackend(tile, pool_bytes=128 << 20), S.OracleBackend()
+    g.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
+    o.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
+    types, keys, lens, vals, offs = _concat(batches, mk, mv)
+    got = _run_multi(g, types, keys, lens, vals, offs, mk, mv)
+    exp_v, exp_l, exp_r = [], [], []
+    for b in batches:
+        v, l, r = o.mixed(b["types"], b["keys"], b["lens"], b["values"],
+                          np.full(len(b["types"]), mv, np.uint32), mk, mv)
+        exp_v.append(v)
+        exp_l.append(l)
+        exp_r.append(r)
+    assert np.array_equal(got[0], np.concatenate(exp_v))
+    assert np.array_equal(got[1], np.concatenate(exp_l))
+    assert np.array_equal(got[2], np.concatenate(exp_r))
+    assert g.items() == o.items()
+
+
+def test_batches_see_each_other_in_order():
+    """insert K / find K / update K / find K / erase K / find K as six batches."""
+    rng = np.random.default_rng(9)
+    keys, lens = W.unique_random_keys(rng, 3000, 1, 5)
+    mk, mv = 5, 2
+    n = len(lens)
+    v1 = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    v2 = v1 ^ np.uint32(0xFFFF)
+    z = np.zeros((n, mv), np.uint32)
+    seq = [(0, v1), (2, z), (3, v2), (2, z), (1, z), (2, z)]
+    batches = [dict(types=np.full(n, t, np.uint8), keys=keys, lens=lens, values=v)
+               for t, v in seq]
+    types, k, l, vals, offs = _concat(batches, mk, mv)
+    g = S.GpuBackend(16, pool_bytes=64 << 20)
+    gv, gl, gr = _run_multi(g, types, k, l, vals, offs, mk, mv)
+    gv = gv.reshape(6, n, mv)
+    gl = gl.reshape(6, n)
+    gr = gr.reshape(6, n)
+    assert gr[0].all() and gr[2].all() and gr[4].all()
+    assert (gl[1] == mv).all() and np.array_equal(gv[1], v1)
+    assert (gl[3] == mv).all() and np.array_equal(gv[3], v2)
+    assert (gl[5] == 0).all()
+    assert g.items() == []
+
+
+def test_tile_width_switches_between_calls():
+    """All tile widths share one tree: insert at 16 lanes, update at 32, find at 64,
+    erase at 32, and every step matches the oracle."""
+    rng = np.random.default_rng(12)
+    keys, lens = W.unique_random_keys(rng, 20_000, 1, 6)
+    mk, mv = 6, 3
+    n = len(lens)
+    v1 = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    vl = rng.integers(1, mv + 1, size=n).astype(np.uint32)
+    g, o = S.GpuBackend(16, pool_bytes=64 << 20), S.OracleBackend()
+    assert g.t.tile_size == 16
+    assert np.array_equal(g.insert(keys, lens, v1, vl, mk, mv), o.insert(keys, lens, v1, vl, mk, mv))
+    g.t.tile_size = 32
+    half = slice(0, n // 2)
+    v2 = v1[half] ^ np.uint32(0x5A5A5A5A)
+    assert np.array_equal(g.update(keys[half], lens[half], v2, vl[half], mk, mv),
+                          o.update(keys[half], lens[half], v2, vl[half], mk, mv))
+    g.t.tile_size = 64
+    assert g.t.tile_size == 64
+    for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        assert np.array_equal(a, b)
+    g.t.tile_size = 32
+    q = slice(n // 4, 3 * n // 4)
+    assert np.array_equal(g.erase(keys[q], lens[q], mk), o.erase(keys[q], lens[q], mk))
+    assert g.items() == o.items()
+    with pytest.raises(Exception):
+        g.t.tile_size = 48
+    assert g.t.tile_size == 32
