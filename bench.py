@@ -49,21 +49,69 @@ def find_bytes(find_lens, hits, maxv):
     return float(streamed + lines)
 
 
-def cpu_baseline(n_keys, seed):
-    """Oracle (std::map replay, one thread) on a bounded sample of the same workload."""
+def random_line_ceiling(torch, log2_lines=28, blocks=8192, lines_per_stream=64, reps=3):
+    """Measured rate of independent random 64-B line reads over a 2^log2_lines-line
+    (16 GiB) buffer: the ceiling of the lookup access pattern on this GPU."""
+    import ctypes
+    from indexingpu_amd import build as B
+    lib = ctypes.CDLL(B.PROBE_PATH)
+    lib.hbm_random_lines.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+    buf = torch.ones((1 << log2_lines) * 16, dtype=torch.int32, device="cuda")
+    sink = torch.zeros(blocks * 256, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def launch():
+        rc = lib.hbm_random_lines(ctypes.c_void_p(buf.data_ptr()), log2_lines, blocks,
+                                  lines_per_stream, ctypes.c_void_p(sink.data_ptr()),
+                                  ctypes.c_void_p(stream.cuda_stream))
+        if rc != 0:
+            raise RuntimeError(f"hbm_random_lines failed ({rc})")
+
+    launch()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record(stream)
+    for _ in range(reps):
+        launch()
+    e.record(stream)
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / reps
+    lines = blocks * 256 // 4 * lines_per_stream
+    del buf, sink
+    torch.cuda.empty_cache()
+    return dict(lines_per_s=lines / (ms / 1e3), gbs=lines * 64 / (ms / 1e3) / 1e9,
+                buffer_gib=(1 << log2_lines) * 64 / (1 << 30), ms_per_launch=ms)
+
+
+def cpu_baseline(n_keys, seed, threads):
+    """CPU baselines on a bounded sample of the same workload shape: the oracle (std::map
+    replay, one thread) and the multi-threaded CPU index (std::map shards behind the
+    reference CPU adapter's per-key shape, `threads` workers), each timed for the insert
+    batch and the find batch; the CPU index's finds are checked against the oracle's."""
     from indexingpu_amd import workloads as W
+    from oracle.cpu_index import CpuIndex
     from oracle.oracle import Oracle
     d = W.plain_batch_workload(n_keys, seed=seed)
     mk, mv = d["max_key_length"], d["max_value_length"]
+    keys, vals = d["keys"].reshape(-1).copy(), d["values"].reshape(-1).copy()
+    fkeys = d["find_keys"].reshape(-1).copy()
     o = Oracle()
     t0 = time.perf_counter()
-    o.insert(d["keys"].reshape(-1).copy(), mk, d["lens"], d["values"].reshape(-1).copy(), mv,
-             None, n_keys, want_results=False)
+    o.insert(keys, mk, d["lens"], vals, mv, None, n_keys, want_results=False)
     t1 = time.perf_counter()
-    o.find(d["find_keys"].reshape(-1).copy(), mk, d["find_lens"], mv, n_keys)
+    ov, ol = o.find(fkeys, mk, d["find_lens"], mv, n_keys)
     t2 = time.perf_counter()
+    c = CpuIndex(shard_bits=10, threads=threads)
+    t3 = time.perf_counter()
+    c.insert(keys, mk, d["lens"], vals, mv, None, n_keys)
+    t4 = time.perf_counter()
+    cv, cl = c.find(fkeys, mk, d["find_lens"], mv, n_keys)
+    t5 = time.perf_counter()
+    same = bool(np.array_equal(cl, ol) and np.array_equal(cv, ov))
     return dict(find_mops=n_keys / (t2 - t1) / 1e6, insert_mops=n_keys / (t1 - t0) / 1e6,
-                seconds=t2 - t0)
+                mt_find_mops=n_keys / (t5 - t4) / 1e6, mt_insert_mops=n_keys / (t4 - t3) / 1e6,
+                mt_same=same, seconds=t5 - t0)
 
 
 def run_mixed(torch, args):
@@ -225,8 +273,9 @@ def run_batched(torch, args):
         return (torch.equal(vl, cat["expect_len"]) and torch.equal(v, cat["expect_values"])
                 and torch.equal(r, cat["expect_results"]))
 
+    o = offs.tolist()  # host copy (no device reads inside a graph capture)
+
     def per_batch_calls(tree, v, vl, r):
-        o = offs.tolist()
         for i in range(len(sizes)):
             a, b = o[i], o[i + 1]
             tree.mixed_batch(cat["types"][a:b], cat["keys"][a:b], mk, cat["lens"][a:b],
@@ -317,8 +366,78 @@ def run_batched(torch, args):
         del v, vl, r
         torch.cuda.empty_cache()
     return dict(batched=dict(preload_keys=n, batches=len(sizes), requests=total,
-                             key_slices=mk, prefixes=256, tuned=best, tune_table=tune_table,
+                             key_slices=mk, prefixes=256, tuned_config=best, tune_table=tune_table,
                              **out))
+
+
+def run_served(torch, args):
+    """Served form against the plain path on one resident index: args.served_keys keys
+    of 2..8 slices (2-slice values), traces of requests of floor(exp(U(0, ln 4096)))
+    operations (80% find / 10% insert / 5% erase / 5% scan, limit 50) with Poisson
+    arrivals at a sweep of rates.  The served loop (continuous batching, waves on 4
+    streams) and the plain path (one mixed-batch call plus one scan call per request,
+    in order) replay traces of the same kind; each point reports Mops/s (operations of
+    completed requests over the run's wall time) and p50/p99/p99.9 latency from arrival
+    to completion.  Point-op results are checked against the generator's expectation;
+    scan counts are checked (every scan lies in a region with >= 50 keys after it, except
+    near its end).  A mode's sweep stops once its p99 passes 20 ms (saturated)."""
+    from indexingpu_amd import GpuMasstree
+    from indexingpu_amd import device_workloads as D
+    from indexingpu_amd.serve import MasstreeServer, latency_summary
+    n = args.served_keys
+    log(f"[served] generating {n} resident keys ...")
+    wl = D.ServeWorkload(n, seed=args.seed + 20)
+    k, l, v = wl.preload()
+    tree = GpuMasstree(tile_size=16, pool_bytes=int(args.served_pool_gb * (1 << 30)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tree.insert(k, wl.mk, l, v, wl.mv, None, n)
+    torch.cuda.synchronize()
+    pre = n / (time.perf_counter() - t0) / 1e6
+    tree.check_device_error()
+    log(f"[served] preload {pre:.1f} Mops/s")
+    rates = [float(x) for x in args.served_rates.split(",")]
+    points = {"served": [], "plain": []}
+    for mode in ("served", "plain"):
+        sv = MasstreeServer(tree, streams=4, max_wave_ops=args.served_wave_ops,
+                            max_wave_scans=4096, ring_rows=1 << 22, epoch_waves=16,
+                            coalesce=(mode == "served"))
+        for rate in rates:
+            nreq = int(min(args.served_max_requests, max(2000, rate * args.served_seconds)))
+            arena, req, expect = wl.trace(nreq, rate)
+            mc = arena["scan_max_count"]
+            n_scans = int(req["scan_offset"][-1] + req["num_scans"][-1])
+            arena = D.ServeWorkload.with_ring(arena, (1 << 22) if mode == "served" else n_scans * mc)
+            out, st = sv.run(arena, req)
+            tree.check_device_error()
+            ok = all(torch.equal(arena[key].view(-1), expect[key].reshape(-1).to(arena[key].dtype))
+                     for key in ("values", "value_lengths", "results"))
+            counts = arena["scan_counts"]
+            ok_scan = bool((counts <= mc).all()) and float((counts == mc).float().mean()) > 0.99
+            ops = int(req["num_ops"].sum() + req["num_scans"].sum())
+            lat = latency_summary(req, out)
+            span_us = float(out["done_us"].max())
+            p = dict(rate_req_s=rate, requests=nreq, ops=ops,
+                     offered_mops=round(ops / (float(req["arrival_us"][-1]) + 1e-9), 2),
+                     mops=round(ops / span_us, 2), waves=st["waves"],
+                     max_inflight=st["max_inflight"], verified=ok and ok_scan,
+                     **{kk: round(vv, 1) for kk, vv in lat.items()})
+            points[mode].append(p)
+            log(f"[served] {mode} rate {rate:.0f}/s: {p['mops']} Mops/s p50 {p['p50_us']:.0f} "
+                f"p99 {p['p99_us']:.0f} p99.9 {p['p999_us']:.0f} us waves {st['waves']} ok {p['verified']}")
+            del arena
+            if lat["p99_us"] > 20000:
+                break
+        sv.close()
+    tree.close()
+
+    def best(pts):
+        good = [p for p in pts if p["p99_us"] <= 1000 and p["verified"]]
+        return max(good, key=lambda p: p["mops"]) if good else None
+    bs, bp = best(points["served"]), best(points["plain"])
+    return dict(served=dict(resident_keys=n, preload_mops=round(pre, 2), points=points,
+                            best_served_at_p99_1ms=bs, best_plain_at_p99_1ms=bp,
+                            ratio=(round(bs["mops"] / bp["mops"], 2) if bs and bp else None)))
 
 
 def main():
@@ -331,6 +450,8 @@ def main():
     ap.add_argument("--pool-gb", type=float, default=24.0)
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ceiling", action="store_true",
+                    help="skip the random-line read ceiling measurement")
     ap.add_argument("--mixed", action="store_true", help="also run 64M-key mixed batches")
     ap.add_argument("--mixed-keys", type=int, default=1 << 26)
     ap.add_argument("--mixed-batch", type=int, default=1 << 20)
@@ -345,6 +466,14 @@ def main():
     ap.add_argument("--batched-pool-gb", type=float, default=32.0)
     ap.add_argument("--batched-tune-unit", type=int, default=16,
                     help="batches per autotuner measurement unit")
+    ap.add_argument("--served", action="store_true",
+                    help="also run the served form against the plain path (rate sweep)")
+    ap.add_argument("--served-keys", type=int, default=1 << 29)
+    ap.add_argument("--served-pool-gb", type=float, default=112.0)
+    ap.add_argument("--served-rates", default="10000,30000,100000,300000,1000000,2000000,4000000")
+    ap.add_argument("--served-seconds", type=float, default=0.25)
+    ap.add_argument("--served-max-requests", type=int, default=1_000_000)
+    ap.add_argument("--served-wave-ops", type=int, default=1 << 18)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a kernel of "
                          "known traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -358,6 +487,7 @@ def main():
 
     import torch
     from indexingpu_amd import GpuMasstree, workloads as W
+    from indexingpu_amd.autotune import Autotuner
 
     if args.calibrate_copy_mib:
         src = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.int32, device="cuda")
@@ -393,7 +523,12 @@ def main():
     log("[bench] insert batch ...")
     t_ins = timed(lambda: tree.insert(keys, mk, lens, vals, mv, None, n))
     tree.check_device_error()
-    log(f"[bench] insert {n / t_ins / 1e3:.1f} Mops/s; find warmup")
+    log(f"[bench] insert {n / t_ins / 1e3:.1f} Mops/s; tuning find (find tile x depth)")
+    # untimed: the autotuner tries tile width x lookups in flight on this batch and
+    # leaves the fastest setting applied
+    tuner = Autotuner(tree, find_tiles=(4, 8, 16), find_depths=(1, 2, 4), trials=1)
+    best = tuner.tune_find(lambda: tree.find(fkeys, mk, flens, out_v, mv, out_l, n), mk)
+    log(f"[bench] find config {best}; warmup")
     for _ in range(args.warmup):
         tree.find(fkeys, mk, flens, out_v, mv, out_l, n)
     torch.cuda.synchronize()
@@ -451,6 +586,7 @@ def main():
                         "(50% hits) on the loaded index",
             "keys": n, "find_batch": n, "erase_batch": n_erase, "max_key_length": mk,
             "max_value_length": mv, "tile_size": args.tile,
+            "find_tile": best["find_tile"], "find_depth": best["find_depth"],
         },
         "insert_mops": round(n / t_ins / 1e3, 2),
         "erase_mops": round(n_erase / t_er / 1e3, 2),
@@ -462,28 +598,44 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": f"batch_kernel<{args.tile}, false> (find)",
+            "kernel": f"narrow_find_kernel<{best['find_tile']}, {best['find_depth']}>",
             "algorithmic_bytes_per_launch": abytes,
         },
+        "find_tuning": [{"find_tile": c["find_tile"], "find_depth": c["find_depth"],
+                         "mops": round(n / ms / 1e3, 1)} for _, _, c, ms in tuner.report()],
     }
     tree.close()
     del keys, lens, vals, fkeys, flens, ekeys, elens, out_v, out_l
     torch.cuda.empty_cache()
 
+    if not args.no_ceiling:
+        log("[bench] random 64-B line read ceiling ...")
+        ceil = random_line_ceiling(torch)
+        result["roofline"]["measured_ceiling_gbs"] = round(ceil["gbs"], 1)
+        result["roofline"]["frac_of_measured_ceiling"] = round(achieved / ceil["gbs"], 4)
+        result["random_line_ceiling"] = {k: round(v, 3) for k, v in ceil.items()}
     if args.mixed:
         result.update(run_mixed(torch, args))
     if args.batched:
         result.update(run_batched(torch, args))
+    if args.served:
+        result.update(run_served(torch, args))
 
     if not args.no_cpu_baseline:
-        log(f"[bench] CPU baseline: oracle on {args.cpu_sample} keys ...")
-        cb = cpu_baseline(args.cpu_sample, seed=args.seed)
+        threads = max(1, min(16, os.cpu_count() or 1))
+        log(f"[bench] CPU baselines on {args.cpu_sample} keys ({threads} threads) ...")
+        cb = cpu_baseline(args.cpu_sample, seed=args.seed, threads=threads)
         result["cpu_baseline"] = {
-            "value": round(cb["find_mops"], 3), "unit": "Mops/s", "cores": 1, "kind": "port",
-            "sample": f"std::map replay (oracle), 1 thread: {args.cpu_sample} inserts then "
-                      f"{args.cpu_sample} finds (50% hits) of the same workload shape; "
-                      f"insert {cb['insert_mops']:.3f} Mops/s; host {platform.processor() or platform.machine()}, "
-                      f"{os.cpu_count()} logical CPUs",
+            "value": round(cb["mt_find_mops"], 3), "unit": "Mops/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{args.cpu_sample} inserts then {args.cpu_sample} finds (50% hits) of "
+                      f"the same workload shape. value = finds of the multi-threaded CPU index "
+                      f"(oracle/cpu_index.cpp: std::map shards behind the reference CPU "
+                      f"adapter's per-key shape, not masstree-beta; {threads} threads; "
+                      f"insert {cb['mt_insert_mops']:.2f} Mops/s; results equal the oracle's: "
+                      f"{cb['mt_same']}). Oracle std::map replay, 1 thread: find "
+                      f"{cb['find_mops']:.3f}, insert {cb['insert_mops']:.3f} Mops/s. Host "
+                      f"{platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs",
         }
     print(json.dumps(result), flush=True)
 

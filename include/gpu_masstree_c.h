@@ -135,8 +135,14 @@ uint64_t gmt_slabs_allocated(gmt_index* index);
 uint64_t gmt_pool_slabs(gmt_index* index);
 
 /* Lookups each tile keeps in flight in read-only find batches (1, 2, 4 or 8; other
- * values select 4).  A tuning knob with no counterpart in the reference. */
+ * values select 4; default 2).  A tuning knob with no counterpart in the reference. */
 int gmt_set_find_depth(gmt_index* index, uint32_t depth);
+
+/* Lanes per lookup of read-only find batches (gmt_find with concurrent = 0): 4, 8 or 16
+ * (each lane holds 16 / find_tile slots of a node; default 8), or 0 for the index's
+ * tile width.  A tuning knob with no counterpart in the reference; results do not
+ * depend on it. */
+int gmt_set_find_tile(gmt_index* index, uint32_t find_tile);
 
 /* Tile width (lanes per operation: 16, 32 or 64) that later calls on this index use.
  * The reference fixes it per instance as the tile_size template argument
@@ -145,6 +151,84 @@ int gmt_set_find_depth(gmt_index* index, uint32_t depth);
  * current width. */
 int gmt_set_tile_size(gmt_index* index, uint32_t tile_size);
 uint32_t gmt_tile_size(gmt_index* index);
+
+/* ------------------------------------------------------------------ served form
+ * No counterpart in the reference (its batch calls are the whole interface; it has
+ * no request loop).  A server replays a trace of requests against the index in real
+ * time with continuous batching: requests are admitted in arrival order under a
+ * per-wave op/scan limit, a free-slab budget and a scan-output ring budget; each wave
+ * is one mixed-batch launch plus one scan launch on one of `streams` HIP streams, so
+ * waves overlap on the device.  coalesce = 0 runs the plain path instead (one request
+ * per gmt_mixed_batch / gmt_scan pair, in order, on one stream) for comparison. */
+typedef struct gmt_server gmt_server;
+
+typedef struct {
+  uint32_t streams;          /* waves in flight (1..16) */
+  uint32_t slabs_per_write;  /* slabs reserved per insert/update at admission */
+  uint64_t max_wave_ops;     /* point ops per wave */
+  uint64_t max_wave_scans;   /* scans per wave */
+  uint64_t ring_rows;        /* scan result ring, rows of scan_max_count results */
+  uint32_t epoch_waves;      /* waves between retire-list flips */
+  uint32_t coalesce;         /* 1: served form; 0: plain path */
+} gmt_serve_config;
+
+/* Device arena of a trace: point ops in gmt_mixed_batch's layout, scans in gmt_scan's
+ * (per-query limits <= scan_max_count), all requests back to back in arrival order.
+ * Scan results go to the ring: request r's rows start at outcome[r].ring_row, each scan
+ * taking scan_max_count rows.  With coalesce = 0 the ring must hold every scan of the
+ * trace (rows are at scan index * scan_max_count). */
+typedef struct {
+  const uint8_t* types;
+  const uint32_t* keys;
+  uint32_t max_key_length;
+  const uint32_t* key_lengths;
+  uint32_t* values;
+  uint32_t max_value_length;
+  uint32_t* value_lengths;
+  uint8_t* results;
+  const uint32_t* scan_lower;
+  const uint32_t* scan_lower_lengths;
+  const uint32_t* scan_upper;          /* nullable: no upper bound */
+  const uint32_t* scan_upper_lengths;  /* nullable */
+  const uint32_t* scan_limits;         /* nullable */
+  uint32_t scan_max_count;
+  uint32_t* scan_counts;
+  uint32_t* ring_keys;
+  uint32_t* ring_key_lengths;
+  uint32_t* ring_values;
+  uint32_t* ring_value_lengths;
+} gmt_serve_arena;
+
+typedef struct {
+  double arrival_us;     /* from the start of the run; non-decreasing */
+  uint64_t op_offset;    /* = previous op_offset + previous num_ops */
+  uint64_t scan_offset;  /* = previous scan_offset + previous num_scans */
+  uint32_t num_ops;
+  uint32_t num_scans;
+  uint32_t num_writes;   /* inserts + updates among the ops */
+} gmt_serve_request;
+
+typedef struct {
+  double admit_us;   /* launch time of its wave */
+  double done_us;    /* completion time of its wave, as seen by the loop */
+  uint64_t ring_row;
+  uint32_t wave;
+} gmt_serve_outcome;
+
+typedef struct {
+  uint64_t waves, ops, scans, flips, drains, census, admission_waits;
+  double elapsed_us;
+  uint32_t max_inflight;
+  uint32_t device_error;  /* gmt_device_error_code seen at the end of the run */
+} gmt_serve_stats;
+
+/* Creates a server for the index at its current tile width. */
+int gmt_server_create(gmt_index* index, const gmt_serve_config* config, gmt_server** out);
+int gmt_server_destroy(gmt_server* server);
+/* Replays n requests (host array) and fills outcomes (host array of n) and stats. */
+int gmt_server_run(gmt_server* server, const gmt_serve_arena* arena,
+                   const gmt_serve_request* requests, uint64_t n, gmt_serve_outcome* outcomes,
+                   gmt_serve_stats* stats);
 
 /* Message of the last failing call on this thread. */
 const char* gmt_last_error_message(void);

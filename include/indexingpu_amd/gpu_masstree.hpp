@@ -22,6 +22,7 @@
 #include "host_walk.hpp"
 #include "kernels.hpp"
 #include "layout.hpp"
+#include "narrow_find.hpp"
 #include "reclaim.hpp"
 #include "slab_alloc.hpp"
 #include "tile_policy.hpp"
@@ -118,7 +119,8 @@ struct gpu_masstree {
   gpu_masstree() = delete;
   gpu_masstree(const host_allocator_type& host_allocator, const host_reclaimer_type& host_reclaimer)
       : allocator_(host_allocator.get_device_instance()),
-        reclaimer_(host_reclaimer.get_device_instance()) {
+        reclaimer_(host_reclaimer.get_device_instance()),
+        inserted_hint_(new uint64_t(0)) {
     allocate();
   }
   // shallow copy (kernels receive the tree by value, as in the reference)
@@ -127,7 +129,11 @@ struct gpu_masstree {
         allocator_(other.allocator_),
         reclaimer_(other.reclaimer_),
         err_(other.err_),
+        barrier_(other.barrier_),
+        inserted_hint_(other.inserted_hint_),
         owner_(false) {}
+  template <typename, typename, uint32_t>
+  friend struct gpu_masstree;
   // Attaches to the tree of an index with another tile width: the node layout does not
   // depend on the tile width, so both operate on the same tree (the owner frees it).
   template <uint32_t other_tile>
@@ -136,12 +142,16 @@ struct gpu_masstree {
         allocator_(other.allocator_),
         reclaimer_(other.reclaimer_),
         err_(other.err_),
+        barrier_(other.barrier_),
+        inserted_hint_(other.inserted_hint_),
         owner_(false) {}
   gpu_masstree& operator=(const gpu_masstree&) = delete;
   ~gpu_masstree() {
+    if (scan_stack_) (void)hipFree(scan_stack_);  // per instance, never shared
     if (owner_) {
       if (err_) (void)hipFree(err_);
-      if (scan_stack_) (void)hipFree(scan_stack_);
+      if (barrier_) (void)hipFree(barrier_);
+      delete inserted_hint_;
     }
   }
 
@@ -160,18 +170,22 @@ struct gpu_masstree {
     if (concurrent) {
       launch_batch<true>(a, stream, false);
     } else {
-      switch (find_depth_) {
-        case 1: launch_find<1>(a, stream); break;
-        case 2: launch_find<2>(a, stream); break;
-        case 8: launch_find<8>(a, stream); break;
-        default: launch_find<4>(a, stream); break;
+      switch (find_tile_) {
+        case 4: launch_narrow<4>(a, stream); break;
+        case 8: launch_narrow<8>(a, stream); break;
+        case 16: launch_narrow<16>(a, stream); break;
+        default: launch_wide(a, stream); break;
       }
     }
   }
 
-  // Lookups kept in flight per tile by the read-only find kernel (1, 2, 4 or 8).
+  // Lookups kept in flight per tile by the read-only find kernels (1, 2, 4 or 8).
   void set_find_depth(uint32_t k) { find_depth_ = (k == 1 || k == 2 || k == 8) ? k : 4; }
   uint32_t find_depth() const { return find_depth_; }
+  // Lanes per lookup of read-only finds: 4, 8 or 16 (narrow tiles, each lane holding
+  // 16 / width slots of a node), or 0 for the index's own tile width.
+  void set_find_tile(uint32_t t) { find_tile_ = (t == 4 || t == 8 || t == 16) ? t : 0; }
+  uint32_t find_tile() const { return find_tile_; }
 
   template <bool update_if_exists = false, bool enable_suffix = true, bool use_shmem_key = false>
   void insert(const key_slice_type* keys, const size_type max_key_length,
@@ -185,7 +199,7 @@ struct gpu_masstree {
     a.max_value_length = max_value_length;
     a.value_lengths = value_lengths;
     a.results = results;
-    launch_batch<true>(a, stream, true);
+    launch_growing(a, stream);
   }
 
   template <bool enable_suffix = true, bool use_shmem_key = false>
@@ -294,12 +308,14 @@ struct gpu_masstree {
     m.a.results = results;
     m.offsets = offsets;
     m.num_batches = num_batches;
-    m.barrier = err_ + 2;
+    m.barrier = barrier_;
     auto kernel = indexingpu_amd::multi_batch_kernel<tile_size>;
     // every block must be resident at once (the kernel has a grid-wide barrier)
     const uint32_t blocks = std::max<uint32_t>(
         1, (uint32_t)((uint64_t)indexingpu_amd::resident_blocks(kernel) * multi_grid_percent_ / 100));
-    hipLaunchKernelGGL(indexingpu_amd::zero_kernel, dim3(1), dim3(64), 0, stream, m.barrier, 1u);
+    m.groups = std::min<uint32_t>(blocks, indexingpu_amd::kMaxBarrierGroups);
+    hipLaunchKernelGGL(indexingpu_amd::zero_kernel, dim3(1), dim3(256), 0, stream, m.barrier,
+                       indexingpu_amd::kBarrierWords);
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
                        view(), m);
     drain(stream);
@@ -437,6 +453,7 @@ struct gpu_masstree {
   device_allocator_instance_type allocator_;
   device_reclaimer_instance_type reclaimer_;
   uint32_t* err_ = nullptr;
+  uint32_t* barrier_ = nullptr;  // grid barrier area of the batched form
 
  private:
   template <class K>
@@ -497,15 +514,63 @@ struct gpu_masstree {
     if (mutating) drain(stream);
   }
 
-  template <uint32_t K>
-  void launch_find(const indexingpu_amd::batch_args& a, hipStream_t stream) {
+  // A large insert batch into a small tree would start every tile on the same few
+  // nodes, where they serialize on the locks and on the coherent reads of those lines
+  // (16M keys into an empty tree: 13 Mops/s in one launch, ~470 Mops/s in growing
+  // launches).  The batch therefore runs as consecutive launches that each grow the
+  // tree at most kGrowthFactor-fold (from kGrowthMin keys), then the rest in one launch.
+  // The requests of one batch are independent of each other, so consecutive launches
+  // give the results of one.  The size estimate counts every key inserted through this
+  // API (nodes are never merged, so it also bounds how spread out the tree is).
+  static constexpr uint64_t kGrowthMin = 512;
+  static constexpr uint64_t kGrowthFactor = 4;
+  void launch_growing(indexingpu_amd::batch_args a, hipStream_t stream) {
+    uint64_t size = *inserted_hint_;
+    uint64_t left = a.n;
+    while (left > 0) {
+      uint64_t stage = std::max<uint64_t>(kGrowthMin, size * (kGrowthFactor - 1));
+      if (stage > left) stage = left;
+      indexingpu_amd::batch_args s = a;
+      s.n = stage;
+      launch_batch<true>(s, stream, true);
+      a.keys += stage * a.max_key_length;
+      if (a.key_lengths) a.key_lengths += stage;
+      if (a.values) a.values += stage * a.max_value_length;
+      if (a.value_lengths) a.value_lengths += stage;
+      if (a.results) a.results += stage;
+      left -= stage;
+      size += stage;
+    }
+    *inserted_hint_ = size;
+  }
+
+  template <typename Kern>
+  void launch_reader(Kern kernel, uint32_t per_block, const indexingpu_amd::batch_args& a,
+                     hipStream_t stream) {
     if (a.n == 0) return;
-    auto kernel = indexingpu_amd::find_kernel<tile_size, K>;
-    const uint32_t per_block = indexingpu_amd::kBlockThreads / tile_size * K;
     uint64_t want = (a.n + per_block - 1) / per_block;
     uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(kernel));
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
                        view(), a);
+  }
+  template <uint32_t T>
+  void launch_narrow(const indexingpu_amd::batch_args& a, hipStream_t stream) {
+    constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / T;
+    switch (find_depth_) {
+      case 1: launch_reader(indexingpu_amd::narrow_find_kernel<T, 1>, tiles, a, stream); break;
+      case 2: launch_reader(indexingpu_amd::narrow_find_kernel<T, 2>, tiles * 2, a, stream); break;
+      case 8: launch_reader(indexingpu_amd::narrow_find_kernel<T, 8>, tiles * 8, a, stream); break;
+      default: launch_reader(indexingpu_amd::narrow_find_kernel<T, 4>, tiles * 4, a, stream); break;
+    }
+  }
+  void launch_wide(const indexingpu_amd::batch_args& a, hipStream_t stream) {
+    constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / tile_size;
+    switch (find_depth_) {
+      case 1: launch_reader(indexingpu_amd::find_kernel<tile_size, 1>, tiles, a, stream); break;
+      case 2: launch_reader(indexingpu_amd::find_kernel<tile_size, 2>, tiles * 2, a, stream); break;
+      case 8: launch_reader(indexingpu_amd::find_kernel<tile_size, 8>, tiles * 8, a, stream); break;
+      default: launch_reader(indexingpu_amd::find_kernel<tile_size, 4>, tiles * 4, a, stream); break;
+    }
   }
 
   void ensure_scan_stack(uint64_t words) {
@@ -519,6 +584,7 @@ struct gpu_masstree {
   void allocate() {
     IGPU_HIP_TRY(hipMalloc(&err_, 256));
     IGPU_HIP_TRY(hipMemset(err_, 0, 256));
+    IGPU_HIP_TRY(hipMalloc(&barrier_, indexingpu_amd::kBarrierWords * 4));
     uint32_t* d_root = err_ + 1;
     hipLaunchKernelGGL(indexingpu_amd::init_root_kernel, dim3(1), dim3(64), 0, 0, allocator_, err_,
                        d_root);
@@ -529,8 +595,10 @@ struct gpu_masstree {
       throw indexingpu_amd::hip_error("gpu_masstree: slab pool exhausted at construction");
   }
 
+  uint64_t* inserted_hint_ = nullptr;  // host counter shared with attached instances
   bool owner_ = true;
-  uint32_t find_depth_ = 4;
+  uint32_t find_depth_ = 2;
+  uint32_t find_tile_ = 8;
   uint32_t multi_grid_percent_ = 100;
   uint32_t* scan_stack_ = nullptr;
   uint64_t scan_stack_words_ = 0;

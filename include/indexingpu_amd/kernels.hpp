@@ -135,16 +135,30 @@ __global__ void __launch_bounds__(kBlockThreads) batch_kernel(tree_view tv, batc
 // Batched form: one persistent launch applies a sequence of mixed batches in order
 // (batch b + 1 sees every effect of batch b).  Requests of all batches are concatenated;
 // offsets[b] .. offsets[b + 1] delimit batch b.  Tiles take the requests of the current
-// batch by static stride; a tile with none left reports to its block (LDS counter), the
-// last tile of the block adds one to a global counter, and a leader tile per block polls
-// that counter and releases the block into the next batch.  The barrier lives in the
-// same flat loop as the operations, so waiting tiles never stall working sibling tiles
-// of their wave.  Records retired by any batch are drained once, after the launch.
+// batch by static stride.  Between batches runs a two-level grid barrier:
+//   * a tile with no request left reports to its block (LDS counter); the block's last
+//     tile adds 1 to its group's arrival counter (block g of the grid is in group
+//     g % groups, each counter on its own 128-B line, so arrivals spread over `groups`
+//     lines instead of queueing on one);
+//   * the block that completes its group adds 1 to the top counter; the block that
+//     completes the top counter stores b + 1 into every group's release line;
+//   * one tile per block polls its group's release line (about grid / groups pollers per
+//     line) and releases the block into the next batch.
+// Counters only grow during a launch, so no reset is needed between batches.  The
+// barrier steps run inside the same flat loop as the operations, so waiting tiles never
+// stall working sibling tiles of their wave.  Records retired by any batch are drained
+// once, after the launch.
+constexpr u32 kMaxBarrierGroups = 64;
+constexpr u32 kLineWords = 32;
+// words of the barrier area: arrival lines, top line, release lines
+constexpr u32 kBarrierWords = (2 * kMaxBarrierGroups + 1) * kLineWords;
+
 struct multi_args {
   batch_args a;         // a.n = total requests of all batches
   const u64* offsets;   // num_batches + 1 request offsets
   u32 num_batches;
-  u32* barrier;         // zero at launch; blocks add 1 per finished batch
+  u32 groups;           // barrier groups, 1 .. kMaxBarrierGroups, <= gridDim.x
+  u32* barrier;         // kBarrierWords words, zero at launch
 };
 
 constexpr u32 kBarrierGuard = 1u << 26;
@@ -234,14 +248,26 @@ __global__ void __launch_bounds__(kBlockThreads) multi_batch_kernel(tree_view tv
           s_done = 0;
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
           dev::order();
-          dev::atomic_add(m.barrier, 1u);
+          const u32 g = blockIdx.x % m.groups;
+          const u32 members = gridDim.x / m.groups + (g < gridDim.x % m.groups ? 1u : 0u);
+          const u32 arrived = dev::atomic_add(m.barrier + g * kLineWords, 1u) + 1u;
+          if (arrived == (b + 1) * members) {
+            u32* top = m.barrier + kMaxBarrierGroups * kLineWords;
+            const u32 done_groups = dev::atomic_add(top, 1u) + 1u;
+            if (done_groups == (b + 1) * m.groups) {
+              dev::order();
+              u32* rel = top + kLineWords;
+              for (u32 k = 0; k < m.groups; k++) dev::coherent_store(rel + k * kLineWords, b + 1);
+            }
+          }
         }
       }
       if (tile_in_block == 0) {
-        u32 g = 0;
-        if (c.lane == 0) g = dev::coherent_load(m.barrier);
-        g = c.bcast(g, 0);
-        if (g >= (b + 1) * gridDim.x && c.lane == 0) s_released = b + 1;
+        const u32* rel = m.barrier + (kMaxBarrierGroups + 1 + blockIdx.x % m.groups) * kLineWords;
+        u32 r = 0;
+        if (c.lane == 0) r = dev::coherent_load(rel);
+        r = c.bcast(r, 0);
+        if (r >= b + 1 && c.lane == 0) s_released = b + 1;
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       const u32 rel = c.bcast(*((volatile u32*)&s_released), 0);

@@ -3,14 +3,16 @@
 The knobs (all switchable between calls on one index, same tree state):
   * tile width: lanes per operation, 16 / 32 / 64 (the reference's tile_size template
     argument, include/gpu_masstree.hpp:42-46, fixed per instance there);
+  * find tile: lanes per lookup of read-only find batches (4/8/16: each lane holds
+    16 / width slots of a node; 0 = the tile width);
   * find depth: lookups each tile keeps in flight in read-only find batches (1/2/4/8);
   * grid share: percent of the resident-block capacity the batched form launches.
 
 Node fanout and slab size class are not knobs here: every node is one 128-byte slab of
 16 eight-byte slots (border nodes hold up to 14 keys, interior nodes up to 15 children),
 the size one 16-lane tile reads in one coalesced access, and records use the same slab
-class.  The tuner therefore searches tile x depth (finds) and tile x grid share
-(batched form).
+class.  The tuner therefore searches find tile x depth (read-only finds) and tile x
+grid share (batched form).
 
 Choices are cached per (operation class, key-length class): the key-length class is the
 maximum key length rounded up to a power of two (1, 2, 4, 8, 16, 32, 64, ...), since the
@@ -27,6 +29,7 @@ import itertools
 import time
 
 TILES = (16, 32, 64)
+FIND_TILES = (4, 8, 16)
 FIND_DEPTHS = (1, 2, 4, 8)
 GRID_PERCENTS = (100, 50, 25)
 
@@ -73,10 +76,11 @@ class Autotuner:
     ``timer(fn) -> ms`` defaults to HIP events on torch's current stream.
     """
 
-    def __init__(self, index, tiles=TILES, find_depths=FIND_DEPTHS,
+    def __init__(self, index, tiles=TILES, find_tiles=FIND_TILES, find_depths=FIND_DEPTHS,
                  grid_percents=GRID_PERCENTS, trials=3, timer=None):
         self.index = index
         self.tiles = tuple(tiles)
+        self.find_tiles = tuple(find_tiles)
         self.find_depths = tuple(find_depths)
         self.grid_percents = tuple(grid_percents)
         self.trials = max(1, int(trials))
@@ -88,6 +92,8 @@ class Autotuner:
     def apply_config(self, cfg):
         if "tile" in cfg:
             self.index.tile_size = cfg["tile"]
+        if "find_tile" in cfg:
+            self.index.set_find_tile(cfg["find_tile"])
         if "find_depth" in cfg:
             self.index.set_find_depth(cfg["find_depth"])
         if "grid_percent" in cfg:
@@ -114,11 +120,11 @@ class Autotuner:
 
     def tune_find(self, run_find, max_key_length, warmup=1):
         """run_find() launches one read-only find batch (the same batch every call).
-        Every (tile, depth) candidate is warmed up, then timed `trials` times; the
+        Every (find tile, depth) candidate is warmed up, then timed `trials` times; the
         minimum counts."""
         scored = []
-        for tile, depth in itertools.product(self.tiles, self.find_depths):
-            cfg = {"tile": tile, "find_depth": depth}
+        for tile, depth in itertools.product(self.find_tiles, self.find_depths):
+            cfg = {"find_tile": tile, "find_depth": depth}
             self.apply_config(cfg)
             for _ in range(warmup):
                 self.timer(run_find)

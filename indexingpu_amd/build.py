@@ -1,4 +1,6 @@
-"""Builds libgpu_masstree.so (HIP, gfx950) in-tree with hipcc."""
+"""Builds the package's HIP libraries (gfx950) in-tree with hipcc:
+libgpu_masstree.so (the index) and libhbm_probe.so (random-line read ceiling used by
+the benchmark)."""
 import os
 import subprocess
 import sys
@@ -7,6 +9,8 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libgpu_masstree.so")
 SOURCES = [os.path.join(PKG_DIR, "csrc", "gpu_masstree_capi.hip")]
+PROBE_PATH = os.path.join(PKG_DIR, "libhbm_probe.so")
+PROBE_SOURCES = [os.path.join(PKG_DIR, "csrc", "hbm_probe.hip")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -25,14 +29,19 @@ def hipcc():
     return cand if os.path.exists(cand) else "hipcc"
 
 
-def build(verbose=False):
-    """Compiles every HIP source of the package into libgpu_masstree.so."""
+def _compile(sources, out, verbose):
     cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(REPO_DIR, "include"), "-o",
-                                      LIB_PATH + ".tmp"] + SOURCES
+                                      out + ".tmp"] + sources
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     subprocess.check_call(cmd)
-    os.replace(LIB_PATH + ".tmp", LIB_PATH)
+    os.replace(out + ".tmp", out)
+
+
+def build(verbose=False):
+    """Compiles every HIP source of the package (both libraries)."""
+    _compile(SOURCES, LIB_PATH, verbose)
+    _compile(PROBE_SOURCES, PROBE_PATH, verbose)
     return LIB_PATH
 
 

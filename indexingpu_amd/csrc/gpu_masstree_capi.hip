@@ -8,6 +8,7 @@
 
 #include "gpu_masstree_c.h"
 #include "indexingpu_amd/gpu_masstree.hpp"
+#include "indexingpu_amd/server.hpp"
 
 namespace {
 
@@ -310,6 +311,14 @@ int gmt_set_find_depth(gmt_index* index, uint32_t depth) {
   return GMT_OK;
 }
 
+int gmt_set_find_tile(gmt_index* index, uint32_t find_tile) {
+  if (!index) return bad("gmt_set_find_tile: null handle");
+  if (find_tile != 0 && find_tile != 4 && find_tile != 8 && find_tile != 16)
+    return bad("gmt_set_find_tile: find_tile must be 0, 4, 8 or 16");
+  B(index)->on_all([&](auto& t) { t.set_find_tile(find_tile); });
+  return GMT_OK;
+}
+
 int gmt_set_tile_size(gmt_index* index, uint32_t tile_size) {
   if (!index) return bad("gmt_set_tile_size: null handle");
   if (!valid_tile(tile_size)) return bad("gmt_set_tile_size: tile_size must be 16, 32 or 64");
@@ -318,6 +327,109 @@ int gmt_set_tile_size(gmt_index* index, uint32_t tile_size) {
 }
 
 uint32_t gmt_tile_size(gmt_index* index) { return index ? B(index)->tile : 0; }
+
+struct gmt_server {
+  std::unique_ptr<indexingpu_amd::masstree_server<tree_t<16>>> s16;
+  std::unique_ptr<indexingpu_amd::masstree_server<tree_t<32>>> s32;
+  std::unique_ptr<indexingpu_amd::masstree_server<tree_t<64>>> s64;
+};
+
+int gmt_server_create(gmt_index* index, const gmt_serve_config* config, gmt_server** out) {
+  if (!index || !config || !out) return bad("gmt_server_create: null argument");
+  *out = nullptr;
+  if (config->max_wave_ops == 0 || config->ring_rows == 0 || config->epoch_waves == 0)
+    return bad("gmt_server_create: max_wave_ops, ring_rows and epoch_waves must be > 0");
+  indexingpu_amd::serve_config c;
+  c.streams = config->streams;
+  c.slabs_per_write = config->slabs_per_write;
+  c.max_wave_ops = config->max_wave_ops;
+  c.max_wave_scans = config->max_wave_scans;
+  c.ring_rows = config->ring_rows;
+  c.epoch_waves = config->epoch_waves;
+  c.coalesce = config->coalesce;
+  gmt_server* sv = new gmt_server();
+  gmt_impl* p = B(index);
+  int rc = guarded([&] {
+    switch (p->tile) {
+      case 32: sv->s32.reset(new indexingpu_amd::masstree_server<tree_t<32>>(*p->t32, c)); break;
+      case 64: sv->s64.reset(new indexingpu_amd::masstree_server<tree_t<64>>(*p->t64, c)); break;
+      default: sv->s16.reset(new indexingpu_amd::masstree_server<tree_t<16>>(*p->t16, c)); break;
+    }
+  });
+  if (rc != GMT_OK) {
+    delete sv;
+    return rc;
+  }
+  *out = sv;
+  return GMT_OK;
+}
+
+int gmt_server_destroy(gmt_server* server) {
+  if (!server) return bad("gmt_server_destroy: null handle");
+  return guarded([&] { delete server; });
+}
+
+int gmt_server_run(gmt_server* server, const gmt_serve_arena* arena,
+                   const gmt_serve_request* requests, uint64_t n, gmt_serve_outcome* outcomes,
+                   gmt_serve_stats* stats) {
+  if (!server || !arena || (n && (!requests || !outcomes)))
+    return bad("gmt_server_run: null argument");
+  if (n && (!arena->types || !arena->keys || !arena->values || !arena->value_lengths ||
+            !arena->results))
+    return bad("gmt_server_run: the arena needs types, keys, values, value_lengths, results");
+  indexingpu_amd::serve_arena a;
+  a.types = arena->types;
+  a.keys = arena->keys;
+  a.max_key_length = arena->max_key_length;
+  a.key_lengths = arena->key_lengths;
+  a.values = arena->values;
+  a.max_value_length = arena->max_value_length;
+  a.value_lengths = arena->value_lengths;
+  a.results = arena->results;
+  a.scan_lower = arena->scan_lower;
+  a.scan_lower_lengths = arena->scan_lower_lengths;
+  a.scan_upper = arena->scan_upper;
+  a.scan_upper_lengths = arena->scan_upper_lengths;
+  a.scan_limits = arena->scan_limits;
+  a.scan_max_count = arena->scan_max_count;
+  a.scan_counts = arena->scan_counts;
+  a.ring_keys = arena->ring_keys;
+  a.ring_key_lengths = arena->ring_key_lengths;
+  a.ring_values = arena->ring_values;
+  a.ring_value_lengths = arena->ring_value_lengths;
+  std::vector<indexingpu_amd::serve_request> rq(n);
+  for (uint64_t i = 0; i < n; i++) {
+    const gmt_serve_request& r = requests[i];
+    if (r.num_scans && (!a.scan_lower || !a.scan_counts || !a.ring_keys || !a.ring_key_lengths ||
+                        !a.ring_values || !a.ring_value_lengths))
+      return bad("gmt_server_run: scans need scan_lower, scan_counts and the ring buffers");
+    rq[i] = indexingpu_amd::serve_request{r.arrival_us, r.op_offset, r.scan_offset, r.num_ops,
+                                          r.num_scans, r.num_writes};
+  }
+  std::vector<indexingpu_amd::serve_outcome> oc(n);
+  indexingpu_amd::serve_stats st;
+  int rc = guarded([&] {
+    if (server->s16) st = server->s16->run(a, rq.data(), n, oc.data());
+    if (server->s32) st = server->s32->run(a, rq.data(), n, oc.data());
+    if (server->s64) st = server->s64->run(a, rq.data(), n, oc.data());
+  });
+  if (rc != GMT_OK) return rc;
+  for (uint64_t i = 0; i < n; i++)
+    outcomes[i] = gmt_serve_outcome{oc[i].admit_us, oc[i].done_us, oc[i].ring_row, oc[i].wave};
+  if (stats) {
+    stats->waves = st.waves;
+    stats->ops = st.ops;
+    stats->scans = st.scans;
+    stats->flips = st.flips;
+    stats->drains = st.drains;
+    stats->census = st.census;
+    stats->admission_waits = st.admission_waits;
+    stats->elapsed_us = st.elapsed_us;
+    stats->max_inflight = st.max_inflight;
+    stats->device_error = st.device_error;
+  }
+  return GMT_OK;
+}
 
 const char* gmt_last_error_message(void) { return g_last_error.c_str(); }
 

@@ -71,6 +71,7 @@ def load_library():
     lib.gmt_multi_batch.argtypes = [vp, P, P, u32, P, P, u32, P, P, P, u32, u64, vp]
     lib.gmt_set_multi_grid_percent.argtypes = [vp, u32]
     lib.gmt_set_tile_size.argtypes = [vp, u32]
+    lib.gmt_set_find_tile.argtypes = [vp, u32]
     lib.gmt_tile_size.argtypes = [vp]
     lib.gmt_tile_size.restype = u32
     lib.gmt_last_error_message.restype = ctypes.c_char_p
@@ -262,6 +263,10 @@ class GpuMasstree:
     @tile_size.setter
     def tile_size(self, t):
         self._check(self._lib.gmt_set_tile_size(self._h, int(t)))
+
+    def set_find_tile(self, t):
+        """Lanes per lookup of read-only find batches: 4, 8, 16, or 0 (the tile width)."""
+        self._check(self._lib.gmt_set_find_tile(self._h, int(t)))
 
     def set_find_depth(self, k):
         """Lookups each tile keeps in flight in read-only find batches (1, 2, 4, 8)."""

@@ -243,3 +243,42 @@ class MixedWorkload:
         lo = rng.integers(0, 1 << 32, size=(n_queries, self.key_len), dtype=np.uint64).astype(U32)
         lim = rng.integers(1, max_scan + 1, size=n_queries).astype(U32)
         return dict(lower=np.ascontiguousarray(lo), limits=lim, max_count=max_scan)
+
+
+# ----------------------------------------------------------------------------- datasets
+def dataset_keys(lines, big_endian=False, append_zero_at_end=False):
+    """Text keys (one per line) as u32 slice rows, as the reference's dataset loader
+    builds them (evaluation/generate_workload.hpp:131-190): a trailing '\\r' is dropped,
+    empty lines are skipped, each key is cut into 4-byte slices whose numeric value puts
+    the first byte highest (zero padded), so numeric slice order is byte order;
+    big_endian=True byte-swaps every slice; append_zero_at_end adds one zero slice to a
+    key whose length is a multiple of 4 and shorter than the longest key.
+    Returns (keys [n, max_len] u32, lengths u32, min_len, max_len)."""
+    raw = []
+    for ln in lines:
+        b = ln.encode() if isinstance(ln, str) else bytes(ln)
+        if b.endswith(b"\r"):
+            b = b[:-1]
+        if b:
+            raw.append(b)
+    if not raw:
+        raise ValueError("dataset has no keys")
+    nbytes = np.array([len(b) for b in raw], np.int64)
+    lens = ((nbytes + 3) // 4).astype(U32)
+    max_len = int(lens.max())
+    buf = np.zeros((len(raw), max_len * 4), np.uint8)
+    for i, b in enumerate(raw):
+        buf[i, :len(b)] = np.frombuffer(b, np.uint8)
+    keys = buf.view(">u4").astype(U32)
+    if big_endian:
+        keys = keys.byteswap()
+    min_len = int(lens.min())
+    if append_zero_at_end:
+        grow = (lens < max_len) & (nbytes == lens.astype(np.int64) * 4)
+        lens = lens + grow.astype(U32)
+    return np.ascontiguousarray(keys), lens, min_len, max_len
+
+
+def load_dataset_file(path, big_endian=False, append_zero_at_end=False):
+    with open(path, "rb") as f:
+        return dataset_keys(f.read().split(b"\n"), big_endian, append_zero_at_end)

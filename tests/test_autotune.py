@@ -8,7 +8,11 @@ class FakeIndex:
         self.tile_size = 16
         self.depth = 4
         self.grid = 100
+        self.ftile = 8
         self.log = []
+
+    def set_find_tile(self, t):
+        self.ftile = t
 
     def set_find_depth(self, k):
         self.depth = k
@@ -27,7 +31,8 @@ class FakeClock:
     def __call__(self, fn):
         fn()
         self.calls += 1
-        return self.cost(self.index.tile_size, self.index.depth, self.index.grid)
+        i = self.index
+        return self.cost(i.tile_size, i.depth, i.grid, i.ftile)
 
 
 def test_length_class():
@@ -37,13 +42,13 @@ def test_length_class():
 
 def test_tune_find_picks_minimum_and_applies_it():
     idx = FakeIndex()
-    cost = lambda t, d, g: abs(t - 32) + abs(d - 2) * 3 + 1.0
+    cost = lambda t, d, g, f: abs(f - 8) + abs(d - 2) * 3 + 1.0
     clock = FakeClock(idx, cost)
     tu = Autotuner(idx, trials=2, timer=clock)
     runs = []
-    cfg = tu.tune_find(lambda: runs.append((idx.tile_size, idx.depth)), max_key_length=8)
-    assert cfg == {"tile": 32, "find_depth": 2}
-    assert (idx.tile_size, idx.depth) == (32, 2)
+    cfg = tu.tune_find(lambda: runs.append((idx.ftile, idx.depth)), max_key_length=8)
+    assert cfg == {"find_tile": 8, "find_depth": 2}
+    assert (idx.ftile, idx.depth) == (8, 2)
     # 3 tiles x 4 depths, each 1 warmup + 2 timed runs
     assert len(runs) == 12 * 3 and clock.calls == 12 * 3
     assert tu.best("find", 7) == cfg          # same length class (8)
@@ -62,7 +67,7 @@ def test_tune_online_scores_per_operation_and_never_reuses_units():
         served.append(n)
         return n
 
-    def cost(t, d, g):
+    def cost(t, d, g, f):
         return served[-1] * (1.0 + abs(t - 64) / 64 + abs(g - 50) / 100)
 
     tu = Autotuner(idx, trials=2, timer=FakeClock(idx, cost))
@@ -74,10 +79,10 @@ def test_tune_online_scores_per_operation_and_never_reuses_units():
 
 def test_apply_restores_cached_choice():
     idx = FakeIndex()
-    tu = Autotuner(idx, timer=FakeClock(idx, lambda t, d, g: t + d))
+    tu = Autotuner(idx, timer=FakeClock(idx, lambda t, d, g, f: f + d))
     tu.tune_find(lambda: None, max_key_length=4)
-    assert (idx.tile_size, idx.depth) == (16, 1)
-    idx.tile_size, idx.depth = 64, 8
+    assert (idx.ftile, idx.depth) == (4, 1)
+    idx.ftile, idx.depth = 16, 8
     assert tu.apply("find", 3)
-    assert (idx.tile_size, idx.depth) == (16, 1)
+    assert (idx.ftile, idx.depth) == (4, 1)
     assert not tu.apply("mixed", 3)
