@@ -470,7 +470,7 @@ def main():
                     help="also run the served form against the plain path (rate sweep)")
     ap.add_argument("--served-keys", type=int, default=1 << 29)
     ap.add_argument("--served-pool-gb", type=float, default=112.0)
-    ap.add_argument("--served-rates", default="10000,30000,100000,300000,1000000,2000000,4000000")
+    ap.add_argument("--served-rates", default="1000,3000,10000,30000,100000,300000,600000,1000000,2000000")
     ap.add_argument("--served-seconds", type=float, default=0.25)
     ap.add_argument("--served-max-requests", type=int, default=1_000_000)
     ap.add_argument("--served-wave-ops", type=int, default=1 << 18)

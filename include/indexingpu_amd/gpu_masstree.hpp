@@ -516,29 +516,56 @@ struct gpu_masstree {
 
   // A large insert batch into a small tree would start every tile on the same few
   // nodes, where they serialize on the locks and on the coherent reads of those lines
-  // (16M keys into an empty tree: 13 Mops/s in one launch, ~470 Mops/s in growing
+  // (16M keys into an empty tree: 13 Mops/s in one launch, ~500 Mops/s in growing
   // launches).  The batch therefore runs as consecutive launches that each grow the
   // tree at most kGrowthFactor-fold (from kGrowthMin keys), then the rest in one launch.
-  // The requests of one batch are independent of each other, so consecutive launches
-  // give the results of one.  The size estimate counts every key inserted through this
-  // API (nodes are never merged, so it also bounds how spread out the tree is).
+  // The launches take the requests in a scattered order (index times a multiplier coprime to n, modulo n), so
+  // each one is a sample of the whole batch: a batch in key order -- or one whose keys
+  // all lie past the tree's current right edge -- then grows every key range at once
+  // instead of sending every tile to the same edge node.  The requests of one batch are
+  // independent, so any order gives the results of one launch.  The size estimate
+  // counts every key inserted through this API (nodes are never merged, so it also
+  // bounds how spread out the tree is).
   static constexpr uint64_t kGrowthMin = 512;
   static constexpr uint64_t kGrowthFactor = 4;
+  // Weyl multiplier: the integer nearest n / golden ratio that is coprime to n, so
+  // j -> j * m mod n is a bijection and any run of consecutive j lands spread evenly
+  // over [0, n) (three-gap theorem).
+  static uint64_t scatter_multiplier(uint64_t n) {
+    if (n <= 2) return n == 2 ? 1 : 0;
+    uint64_t m = (uint64_t)((double)n * 0.6180339887498949);
+    if (m == 0) m = 1;
+    auto gcd = [](uint64_t x, uint64_t y) {
+      while (y) {
+        const uint64_t t = x % y;
+        x = y;
+        y = t;
+      }
+      return x;
+    };
+    while (gcd(m, n) != 1) m++;
+    return m;
+  }
   void launch_growing(indexingpu_amd::batch_args a, hipStream_t stream) {
     uint64_t size = *inserted_hint_;
-    uint64_t left = a.n;
-    while (left > 0) {
+    const uint64_t n = a.n;
+    if (n == 0) return;
+    if (std::max<uint64_t>(kGrowthMin, size * (kGrowthFactor - 1)) >= n) {
+      launch_batch<true>(a, stream, true);  // the tree is large enough: one launch
+      *inserted_hint_ = size + n;
+      return;
+    }
+    a.perm_n = n;
+    a.perm_mul = scatter_multiplier(n);
+    uint64_t done = 0;
+    while (done < n) {
       uint64_t stage = std::max<uint64_t>(kGrowthMin, size * (kGrowthFactor - 1));
-      if (stage > left) stage = left;
+      if (stage > n - done) stage = n - done;
       indexingpu_amd::batch_args s = a;
+      s.perm_lo = done;
       s.n = stage;
       launch_batch<true>(s, stream, true);
-      a.keys += stage * a.max_key_length;
-      if (a.key_lengths) a.key_lengths += stage;
-      if (a.values) a.values += stage * a.max_value_length;
-      if (a.value_lengths) a.value_lengths += stage;
-      if (a.results) a.results += stage;
-      left -= stage;
+      done += stage;
       size += stage;
     }
     *inserted_hint_ = size;

@@ -20,6 +20,9 @@ namespace indexingpu_amd {
 
 constexpr u32 kBlockThreads = 256;
 constexpr u32 kMaxBackoffShift = 8;
+// A tile that holds locks (a split waiting for its parent) backs off far less: its
+// locks keep other tiles waiting, but retrying at once would hammer the parent's line.
+constexpr u32 kHolderBackoffShift = 0;
 
 struct tree_view {
   slab_pool_view pool;
@@ -45,7 +48,18 @@ struct batch_args {
   u32* out_value_lengths;
   bool* results;
   u64 n;
+  // Optional processing order (batch_kernel): request index i of the launch is request
+  // (perm_lo + i) * perm_mul % perm_n of the arrays; perm_mul = 0 means i itself.  With
+  // perm_mul coprime to perm_n this is a bijection, so consecutive launches over
+  // [0, perm_n) in stages visit every request once, each stage a spread-out sample.
+  u64 perm_lo;
+  u64 perm_mul;
+  u64 perm_n;
 };
+
+__device__ __forceinline__ u64 request_index(const batch_args& a, u64 i) {
+  return a.perm_mul ? (a.perm_lo + i) * a.perm_mul % a.perm_n : i;
+}
 
 template <u32 T>
 __device__ __forceinline__ void load_request(const batch_args& a, u64 i, request& rq) {
@@ -65,17 +79,23 @@ __device__ __forceinline__ bool abort_pending(u32* err) {
   return e == kErrStepLimit || e == kErrBadIndex;
 }
 
+// Six waves per SIMD (at most 80 VGPRs; the step machine alone would take 93 and fit
+// five): the operations are latency bound, and the sixth wave measured about 10% more
+// inserts, finds, erases and mixed operations per second.
 template <u32 T, bool Validate>
-__global__ void __launch_bounds__(kBlockThreads) batch_kernel(tree_view tv, batch_args a) {
+__global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6)))
+batch_kernel(tree_view tv, batch_args a) {
   wave_tile<T, Validate> c(tv.pool, tv.retire, tv.err);
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
   u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
+  u32 ri = 0;  // request_index(a, i); batch launches hold fewer than 2^32 requests
   op_state<gpu_node> st;
   request rq;
   bool active = i < a.n;
   auto start = [&]() {
-    load_request<T>(a, i, rq);
+    ri = (u32)request_index(a, i);
+    load_request<T>(a, ri, rq);
     const bool writes = rq.kind == op_insert || rq.kind == op_update || rq.kind == op_upsert;
     const bool bad_len = rq.key_length == 0 || rq.key_length > a.max_key_length ||
                          (writes && rq.value_length > a.max_value_length);
@@ -102,10 +122,11 @@ __global__ void __launch_bounds__(kBlockThreads) batch_kernel(tree_view tv, batc
       progressed = false;
     } else if (st.phase != ph_done) {
       progressed = op_step<Validate>(c, rq, st);
-      if (progressed || op_holds_locks(st)) {
+      if (progressed) {
         backoff = 0;
       } else {
-        if (backoff < kMaxBackoffShift) backoff += 1;
+        const u32 cap = op_holds_locks(st) ? kHolderBackoffShift : kMaxBackoffShift;
+        backoff = backoff < cap ? backoff + 1 : cap;
         const u32 span = 1u << backoff;
         const u32 jitter = (u32)(i * 0x9E3779B1u) ^ (st.steps * 0x85EBCA6Bu);
         wait = span - 1 + ((jitter >> 7) & (span - 1));
@@ -116,9 +137,9 @@ __global__ void __launch_bounds__(kBlockThreads) batch_kernel(tree_view tv, batc
       wait = 0;
       if (c.lane == 0) {
         if (rq.kind == op_find) {
-          if (a.out_value_lengths) a.out_value_lengths[i] = st.result;
+          if (a.out_value_lengths) a.out_value_lengths[ri] = st.result;
         } else if (a.results) {
-          a.results[i] = st.result != 0;
+          a.results[ri] = st.result != 0;
         }
       }
       i += ntiles;
@@ -213,10 +234,11 @@ __global__ void __launch_bounds__(kBlockThreads) multi_batch_kernel(tree_view tv
         progressed = false;
       } else if (st.phase != ph_done) {
         progressed = op_step<true>(c, rq, st);
-        if (progressed || op_holds_locks(st)) {
+        if (progressed) {
           backoff = 0;
         } else {
-          if (backoff < kMaxBackoffShift) backoff += 1;
+          const u32 cap = op_holds_locks(st) ? kHolderBackoffShift : kMaxBackoffShift;
+          backoff = backoff < cap ? backoff + 1 : cap;
           const u32 span = 1u << backoff;
           const u32 jitter = (u32)(i * 0x9E3779B1u) ^ (st.steps * 0x85EBCA6Bu);
           wait = span - 1 + ((jitter >> 7) & (span - 1));

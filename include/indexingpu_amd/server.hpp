@@ -117,6 +117,8 @@ class masstree_server {
  public:
   masstree_server(Tree& tree, const serve_config& cfg) : tree_(tree), cfg_(cfg) {
     cfg_.streams = std::max<uint32_t>(1, std::min<uint32_t>(cfg_.streams, 16));
+    // one wave is one batch launch, whose request indices are 32-bit
+    cfg_.max_wave_ops = std::min<uint64_t>(cfg_.max_wave_ops, 1ull << 31);
     const uint32_t nstreams = cfg_.coalesce ? cfg_.streams : 1u;
     for (uint32_t s = 0; s < nstreams; s++) {
       hipStream_t st;

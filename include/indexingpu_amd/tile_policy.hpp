@@ -78,6 +78,9 @@ struct wave_tile {
     if (lane == 0) dev::set_error(err, code);
   }
   __device__ __forceinline__ bool failed() const { return fatal; }
+  // coarse device time for the per-operation time guard: the 100 MHz wall clock in
+  // units of 1024 ticks (about 10 us)
+  __device__ __forceinline__ u32 coarse_clock() const { return (u32)(wall_clock64() >> 10); }
   __device__ __forceinline__ bool check(u32 idx) {
     if (idx < pool.num_slabs) return true;
     set_error(kErrBadIndex);

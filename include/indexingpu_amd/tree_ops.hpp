@@ -75,13 +75,21 @@ struct op_state {
   u32 snap_hdr;    // header of `img` when it was validated (kInvalid = must reload)
   u32 a0, a1, a2, a3;  // split propagation: left, right, separator, parent level
   u32 result;      // find: value length; writes: 1 = success
-  u32 steps;
+  u32 steps;       // steps and waiting iterations so far
+  u32 t0;          // coarse clock when the time guard started (0 = not yet)
   u32 hint[4];     // last interior node visited at levels 1..4 of the current layer
   u32 lvl;         // expected level of `node` (kInvalid: unknown)
   NodeImage img;   // last validated image of `node`
 };
 
-constexpr u32 kMaxStepsPerOp = 1u << 22;
+// Livelock guard.  An operation may legitimately wait a long time behind a hot node
+// (thousands of tiles queue on it), so the guard is time based: after kClockCheckSteps
+// steps-or-waiting-iterations the operation starts a clock, and kMaxOpTime coarse ticks
+// (about 21 s) later it is abandoned with kErrStepLimit.  kMaxStepsPerOp bounds the
+// count itself.
+constexpr u32 kClockCheckSteps = 1u << 16;
+constexpr u32 kMaxOpTime = 1u << 21;
+constexpr u32 kMaxStepsPerOp = 1u << 31;
 constexpr u32 kHintLevels = 4;
 
 // true while the operation holds node locks (a split waiting for its parent): such a
@@ -121,6 +129,7 @@ IGPU_HD void op_begin(C& c, S& st, u32 root) {
   st.a0 = st.a1 = st.a2 = st.a3 = 0;
   st.result = 0;
   st.steps = 0;
+  st.t0 = 0;
   clear_hints(st);
   st.phase = ph_descend;
   (void)c;
@@ -164,10 +173,14 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
     op_finish(st, 0);
     return true;
   }
-  if (st.steps > kMaxStepsPerOp) {
-    c.set_error(kErrStepLimit);
-    op_finish(st, 0);
-    return true;
+  if (st.steps > kClockCheckSteps) {
+    const u32 now = c.coarse_clock() | 1u;
+    if (st.t0 == 0) st.t0 = now;
+    if (now - st.t0 > kMaxOpTime || st.steps > kMaxStepsPerOp) {
+      c.set_error(kErrStepLimit);
+      op_finish(st, 0);
+      return true;
+    }
   }
   const u32 q = c.key_slice(rq, st.layer);
   const bool more = st.layer + 1 < rq.key_length;
@@ -256,12 +269,16 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
     case ph_lock: {
       // Lock only the version we validated: a node that changed since is re-read
       // optimistically (it may have split or become interior) instead of being locked
-      // just to find that out, which would serialize every waiter on a hot node.
+      // just to find that out, which would serialize every waiter on a hot node.  A
+      // changed node means another writer got there first, so the re-read waits out a
+      // backoff like a busy lock: retrying at once would have every waiter re-read and
+      // race for the lock after each release of a hot node (a batch whose keys all land
+      // past the current right edge of the tree ran 5-10x slower that way).
       const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr);
       if (r == kLockBusy) return false;
       if (r == kLockStale) {
         st.phase = ph_descend;
-        return true;
+        return false;
       }
       node_t& n = st.img;
       // the in-memory header carries the lock bit while we hold the lock; keep the image
@@ -480,7 +497,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       if (r == kLockBusy) return false;
       if (r == kLockStale) {  // re-read the candidate (it may have split or grown)
         st.phase = ph_parent_descend;
-        return true;
+        return false;
       }
       node_t& n = st.img;
       const u32 h = st.snap_hdr | kLockBit;

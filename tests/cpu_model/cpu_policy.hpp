@@ -6,6 +6,7 @@
 // acting as a tile) can run operations concurrently against one pool.
 #pragma once
 #include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -46,6 +47,12 @@ struct cpu_tile {
   }
   bool fatal = false;
   bool failed() const { return fatal; }
+  // same unit as the device policy: 10.24 us
+  u32 coarse_clock() const {
+    const auto ns = std::chrono::duration_cast<std::chrono::nanoseconds>(
+                        std::chrono::steady_clock::now().time_since_epoch()).count();
+    return (u32)((uint64_t)ns / 10240u);
+  }
   bool check(u32 idx) {
     if (idx >= P.num_slabs) {
       set_error(kErrBadIndex);

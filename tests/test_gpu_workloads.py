@@ -150,3 +150,29 @@ def test_empty_batches_and_tiny_pool_error():
     r = g._zeros(len(lens), np.uint8)
     g.t.insert(g._d(keys), 3, g._d(lens), g._d(vals), 2, None, len(lens), results=r)
     assert g.t.device_error() == 1  # pool exhausted is reported, not a hang
+
+
+@pytest.mark.parametrize("order", ["ascending", "descending", "two_ranges"])
+def test_key_ordered_insert_batches(order):
+    """Insert batches in key order (and one whose second half lies past everything in
+    the first) grow the tree through the scattered growth stages; every insert succeeds
+    once and the entries equal the oracle's."""
+    rng = np.random.default_rng(77)
+    keys, lens = W.unique_random_keys(rng, 300_000, 1, 4)
+    mk, mv = 4, 2
+    tup = [tuple(keys[i, :lens[i]].tolist()) for i in range(len(lens))]
+    idx = np.array(sorted(range(len(lens)), key=lambda i: tup[i]))
+    if order == "descending":
+        idx = idx[::-1]
+    elif order == "two_ranges":
+        half = len(idx) // 2
+        idx = np.concatenate([rng.permutation(idx[:half]), rng.permutation(idx[half:])])
+    keys, lens = np.ascontiguousarray(keys[idx]), lens[idx].copy()
+    vals = W.make_values(keys, lens, mv)
+    g, o = S.GpuBackend(16, pool_bytes=128 << 20), S.OracleBackend()
+    r = g.insert(keys, lens, vals, None, mk, mv)
+    assert r.all()
+    assert np.array_equal(r, o.insert(keys, lens, vals, None, mk, mv))
+    assert g.items() == o.items()
+    ok, *_ = g.t.validate()
+    assert ok
