@@ -567,6 +567,18 @@ def main():
     abytes = find_bytes(d["find_lens"], n // 2, mv)
     achieved = abytes / (mean_ms / 1e3) / 1e9
 
+    # HBM traffic per launch of this kernel on this workload, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated by a 1 GiB copy); null when the
+    # tuned kernel or the workload differ from the profiled ones
+    traffic, traffic_note = None, None
+    tpath = os.path.join(ROOT, "profiles", "find_kernel_16M_varlen_traffic.json")
+    kname = f"narrow_find_kernel<{best['find_tile']}, {best['find_depth']}>"
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if t["kernel"] == kname and n == (1 << 24) and args.seed == 2:
+            traffic = t["traffic_bytes_per_launch"]
+            traffic_note = ("profiles/find_kernel_16M_varlen_pmc.md: FETCH_SIZE x 1024 x "
+                            f"{t['read_scale']} + WRITE_SIZE x 1024 x {t['write_scale']}")
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -597,9 +609,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "kernel": f"narrow_find_kernel<{best['find_tile']}, {best['find_depth']}>",
             "algorithmic_bytes_per_launch": abytes,
+            "traffic_source": traffic_note,
         },
         "find_tuning": [{"find_tile": c["find_tile"], "find_depth": c["find_depth"],
                          "mops": round(n / ms / 1e3, 1)} for _, _, c, ms in tuner.report()],
