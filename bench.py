@@ -241,12 +241,15 @@ def run_batched(torch, args):
     args.batched_keys preloaded, then args.batched_count uneven mixed batches
     (floor(exp(U(ln 1K, ln 1M))) ops, 90% Zipf finds / 5% fresh inserts / 5% erases of
     live keys not read in the batch).  The same batch sequence runs on a freshly
-    preloaded index in four ways:
+    preloaded index in five ways:
       plain  - one mixed_batch call per batch, launched eagerly;
       graph  - the same calls captured once into a hipGraph and replayed;
-      multi  - one batched-form launch over all batches (tile 16, full grid);
-      tuned  - the batched form with the autotuner's tile width and grid share, tuned
-               beforehand on a separate workload of the same distribution.
+      multi_launches - the batched form, one launch per batch reading its range on the
+               device, one reclamation pass at the end (tile 16, full grid);
+      multi_kernel - the batched form as one persistent launch with a grid barrier
+               between batches;
+      tuned  - the batched form with the autotuner's mode, tile width and grid share,
+               tuned beforehand on a separate workload of the same distribution.
     Every request's output of every way is checked against the generator's model."""
     from indexingpu_amd import GpuMasstree
     from indexingpu_amd import device_workloads as D
@@ -297,7 +300,7 @@ def run_batched(torch, args):
         return s.elapsed_time(e)
 
     # autotune the batched form on a separate workload of the same distribution
-    log("[batched] tuning tile width x grid share ...")
+    log("[batched] tuning mode x tile width x grid share ...")
     twl = D.PrefixBatchWorkload(n, seed=args.seed + 9)
     tk, tl, tv = twl.preload()
     ttree = GpuMasstree(tile_size=16, pool_bytes=pool)
@@ -306,9 +309,12 @@ def run_batched(torch, args):
     tsizes = iter(D.log_uniform_sizes(rng, 4096, 1024, 1 << 20))
     per_unit = args.batched_tune_unit
 
+    space = [{"multi_mode": mode, "tile": t, "grid_percent": g}
+             for mode in (1, 0) for t in (16, 32, 64) for g in (100, 50)]
+
     def tune_units():
         units = []
-        for _ in range(3 * 3 * 2):
+        for _ in range(len(space) * 2):
             c = D.concat_batches([twl.batch(int(next(tsizes))) for _ in range(per_unit)])
             units.append(c)
         return iter(units)
@@ -327,10 +333,9 @@ def run_batched(torch, args):
         return m
 
     tuner = Autotuner(ttree, trials=2)
-    best = tuner.tune_online("multi", run_unit, mk)
+    best = tuner.tune_online("multi", run_unit, mk, space=space)
     ttree.check_device_error()
-    tune_table = [{"tile": c["tile"], "grid_percent": c["grid_percent"],
-                   "mops": round(1e3 / ms / 1e6, 2) if ms > 0 else None}
+    tune_table = [dict(c, mops=round(1e3 / ms / 1e6, 2) if ms > 0 else None)
                   for _, _, c, ms in tuner.report()]
     ttree.close()
     del units
@@ -338,7 +343,7 @@ def run_batched(torch, args):
     log(f"[batched] tuned: {best}")
 
     out = {}
-    for way in ("plain", "graph", "multi", "tuned"):
+    for way in ("plain", "graph", "multi_launches", "multi_kernel", "tuned"):
         tree = GpuMasstree(tile_size=16, pool_bytes=pool)
         pre = _preload(torch, tree, pkeys, plens, pvals, mk, mv)
         v, vl, r = fresh_outputs()
@@ -349,11 +354,13 @@ def run_batched(torch, args):
             with torch.cuda.graph(g):
                 per_batch_calls(tree, v, vl, r)
             ms = timed_ms(g.replay)
-        elif way == "multi":
+        elif way in ("multi_launches", "multi_kernel"):
+            tree.set_multi_mode(1 if way == "multi_launches" else 0)
             ms = timed_ms(lambda: multi_call(tree, v, vl, r))
         else:
             tree.tile_size = best["tile"]
             tree.set_multi_grid_percent(best["grid_percent"])
+            tree.set_multi_mode(best["multi_mode"])
             ms = timed_ms(lambda: multi_call(tree, v, vl, r))
         tree.check_device_error()
         ok = verified(v, vl, r)

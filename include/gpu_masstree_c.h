@@ -99,18 +99,23 @@ int gmt_mixed_batch(gmt_index* index, const uint8_t* types, const uint32_t* keys
                     uint32_t num_requests, void* stream);
 
 /* Batched form (no counterpart in the reference; it extends mixed_batch): applies
- * num_batches mixed batches in order within one persistent launch.  Requests of all
- * batches are stored back to back with gmt_mixed_batch's layout; `offsets` is a device
- * array of num_batches + 1 request offsets; total_requests = offsets[num_batches]. */
+ * num_batches mixed batches in order (batch b + 1 sees every effect of batch b) with one
+ * call and no host round trip.  Requests of all batches are stored back to back with
+ * gmt_mixed_batch's layout; `offsets` is a device array of num_batches + 1 request
+ * offsets; total_requests = offsets[num_batches].  See gmt_set_multi_mode. */
 int gmt_multi_batch(gmt_index* index, const uint8_t* types, const uint32_t* keys,
                     uint32_t max_key_length, const uint32_t* key_lengths, uint32_t* values,
                     uint32_t max_value_length, uint32_t* value_lengths, uint8_t* results,
                     const uint64_t* offsets, uint32_t num_batches, uint64_t total_requests,
                     void* stream);
 
-/* Tuning knob of gmt_multi_batch: share (1..100 %) of the resident-block capacity the
- * persistent launch uses. */
+/* Tuning knobs of gmt_multi_batch: share (1..100 %) of the resident-block capacity its
+ * launches use, and how it runs: mode 0 = one persistent launch with a grid barrier
+ * between batches; mode 1 (default) = one launch per batch, each reading its range from
+ * `offsets` on the device, with one reclamation pass at the end (total_requests must be
+ * below 2^32, else mode 0 is used).  Results do not depend on either knob. */
 int gmt_set_multi_grid_percent(gmt_index* index, uint32_t percent);
+int gmt_set_multi_mode(gmt_index* index, uint32_t mode);
 
 /* Waits for the device and returns (then clears, if clear != 0) the first device-side
  * error code since the last clear (gmt_device_error_code). */

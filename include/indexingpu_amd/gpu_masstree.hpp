@@ -306,6 +306,23 @@ struct gpu_masstree {
     m.a.out_values = values;
     m.a.out_value_lengths = value_lengths;
     m.a.results = results;
+    if (multi_mode_ == 1 && total_requests < (1ull << 32)) {
+      // one batch-kernel launch per batch, each reading its range from `offsets` on the
+      // device; stream order separates the batches, and the records they retire are
+      // drained once at the end (no launch of this call reads a record after it)
+      m.a.results = results;
+      auto kernel = indexingpu_amd::batch_kernel<tile_size, true>;
+      const uint32_t blocks = std::max<uint32_t>(
+          1, (uint32_t)((uint64_t)indexingpu_amd::resident_blocks(kernel) * multi_grid_percent_ / 100));
+      for (uint32_t b = 0; b < num_batches; b++) {
+        indexingpu_amd::batch_args a = m.a;
+        a.range = offsets + b;
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
+                           view(), a);
+      }
+      drain(stream);
+      return;
+    }
     m.offsets = offsets;
     m.num_batches = num_batches;
     m.barrier = barrier_;
@@ -320,6 +337,9 @@ struct gpu_masstree {
                        view(), m);
     drain(stream);
   }
+  // Execution of the batched form: 0 = one persistent launch with a grid barrier between
+  // batches, 1 = one launch per batch (no barrier, one drain at the end).
+  void set_multi_mode(uint32_t mode) { multi_mode_ = mode ? 1u : 0u; }
   // Fraction (percent) of the resident-block capacity the batched form launches.
   void set_multi_grid_percent(uint32_t p) { multi_grid_percent_ = std::min<uint32_t>(std::max<uint32_t>(p, 1), 100); }
 
@@ -627,6 +647,7 @@ struct gpu_masstree {
   uint32_t find_depth_ = 2;
   uint32_t find_tile_ = 8;
   uint32_t multi_grid_percent_ = 100;
+  uint32_t multi_mode_ = 1;
   uint32_t* scan_stack_ = nullptr;
   uint64_t scan_stack_words_ = 0;
 };

@@ -55,6 +55,10 @@ struct batch_args {
   u64 perm_lo;
   u64 perm_mul;
   u64 perm_n;
+  // Optional request range read on the device (batched form, one launch per batch):
+  // when set, the launch covers requests [range[0], range[1]) clamped to n, and the
+  // grid need not match the batch size.
+  const u64* range;
 };
 
 __device__ __forceinline__ u64 request_index(const batch_args& a, u64 i) {
@@ -89,10 +93,15 @@ batch_kernel(tree_view tv, batch_args a) {
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
   u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
+  u64 hi = a.n;
+  if (a.range) {
+    hi = min(a.range[1], a.n);
+    i += min(a.range[0], hi);
+  }
   u32 ri = 0;  // request_index(a, i); batch launches hold fewer than 2^32 requests
   op_state<gpu_node> st;
   request rq;
-  bool active = i < a.n;
+  bool active = i < hi;
   auto start = [&]() {
     ri = (u32)request_index(a, i);
     load_request<T>(a, ri, rq);
@@ -143,7 +152,7 @@ batch_kernel(tree_view tv, batch_args a) {
         }
       }
       i += ntiles;
-      active = i < a.n;
+      active = i < hi;
       if (active) start();
     }
     // every tile of this wave is waiting: let the other waves of the SIMD run
@@ -157,14 +166,13 @@ batch_kernel(tree_view tv, batch_args a) {
 // (batch b + 1 sees every effect of batch b).  Requests of all batches are concatenated;
 // offsets[b] .. offsets[b + 1] delimit batch b.  Tiles take the requests of the current
 // batch by static stride.  Between batches runs a two-level grid barrier:
-//   * a tile with no request left reports to its block (LDS counter); the block's last
-//     tile adds 1 to its group's arrival counter (block g of the grid is in group
-//     g % groups, each counter on its own 128-B line, so arrivals spread over `groups`
-//     lines instead of queueing on one);
-//   * the block that completes its group adds 1 to the top counter; the block that
+//   * a tile with no request left adds 1 to its group's arrival counter (the tiles of
+//     block g of the grid are in group g % groups, each counter on its own 128-B line,
+//     so arrivals spread over `groups` lines instead of queueing on one);
+//   * the tile that completes its group adds 1 to the top counter; the one that
 //     completes the top counter stores b + 1 into every group's release line;
-//   * one tile per block polls its group's release line (about grid / groups pollers per
-//     line) and releases the block into the next batch.
+//   * waiting tiles poll their group's release line and move to the next batch.
+// No LDS is used: the kernel keeps the batch kernel's register and LDS budget.
 // Counters only grow during a launch, so no reset is needed between batches.  The
 // barrier steps run inside the same flat loop as the operations, so waiting tiles never
 // stall working sibling tiles of their wave.  Records retired by any batch are drained
@@ -182,17 +190,14 @@ struct multi_args {
   u32* barrier;         // kBarrierWords words, zero at launch
 };
 
-constexpr u32 kBarrierGuard = 1u << 26;
+constexpr u32 kPollEvery = 8;                         // barrier polls: one iteration in 8
+constexpr u64 kBarrierTicks = 20ull * 100000000ull;   // 20 s at the 100 MHz wall clock
 
+// The batch index and its bounds are uniform over a wave (its tiles advance together),
+// so they are kept in scalar registers.
 template <u32 T>
-__global__ void __launch_bounds__(kBlockThreads) multi_batch_kernel(tree_view tv, multi_args m) {
-  __shared__ u32 s_done;      // tiles of this block finished with the current batch
-  __shared__ u32 s_released;  // batches known complete grid-wide
-  if (threadIdx.x == 0) {
-    s_done = 0;
-    s_released = 0;
-  }
-  __syncthreads();
+__global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5)))
+multi_batch_kernel(tree_view tv, multi_args m) {
   wave_tile<T, true> c(tv.pool, tv.retire, tv.err);
   const batch_args& a = m.a;
   const u32 tiles_per_block = kBlockThreads / T;
@@ -201,14 +206,15 @@ __global__ void __launch_bounds__(kBlockThreads) multi_batch_kernel(tree_view tv
   const u64 gtile = (u64)blockIdx.x * tiles_per_block + tile_in_block;
   op_state<gpu_node> st;
   request rq;
-  u32 b = 0;
+  u32 b = 0;  // current batch (wave-uniform)
   // offsets are clamped to a.n so a malformed offsets array cannot index past the
   // request arrays
-  u64 i = min(m.offsets[0], a.n) + gtile;
   u64 end = min(m.offsets[1], a.n);
+  u64 i = min(m.offsets[0], a.n) + gtile;
   bool have = i < end;
   bool marked = false;
-  u32 backoff = 0, wait = 0, idle = 0;
+  u32 backoff = 0, wait = 0;  // backoff wait while working; barrier iterations while not
+  u64 wait_t0 = 0;            // wall clock when a long barrier wait started
   auto start = [&]() {
     load_request<T>(a, i, rq);
     const bool writes = rq.kind == op_insert || rq.kind == op_update || rq.kind == op_upsert;
@@ -260,18 +266,17 @@ __global__ void __launch_bounds__(kBlockThreads) multi_batch_kernel(tree_view tv
       }
     } else {
       progressed = false;
+      const u32 g = blockIdx.x % m.groups;
       if (!marked) {
+        // arrive: one count per tile on the group's line; the tile that completes its
+        // group counts on the top line, and the one that completes the top line
+        // releases every group
         marked = true;
-        idle = 0;
-        u32 v = 0;
-        if (c.lane == 0) v = atomicAdd(&s_done, 1u) + 1u;
-        v = c.bcast(v, 0);
-        if (v == tiles_per_block && c.lane == 0) {
-          s_done = 0;
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-          dev::order();
-          const u32 g = blockIdx.x % m.groups;
-          const u32 members = gridDim.x / m.groups + (g < gridDim.x % m.groups ? 1u : 0u);
+        wait = 0;
+        if (c.lane == 0) {
+          dev::order();  // this tile's tree writes are complete before it is counted
+          const u32 members =
+              (gridDim.x / m.groups + (g < gridDim.x % m.groups ? 1u : 0u)) * tiles_per_block;
           const u32 arrived = dev::atomic_add(m.barrier + g * kLineWords, 1u) + 1u;
           if (arrived == (b + 1) * members) {
             u32* top = m.barrier + kMaxBarrierGroups * kLineWords;
@@ -284,28 +289,36 @@ __global__ void __launch_bounds__(kBlockThreads) multi_batch_kernel(tree_view tv
           }
         }
       }
-      if (tile_in_block == 0) {
-        const u32* rel = m.barrier + (kMaxBarrierGroups + 1 + blockIdx.x % m.groups) * kLineWords;
+      // Poll the group's release line on one iteration in kPollEvery: the tiles of a
+      // wave share the line (one request per wave), so they read one value and advance
+      // together, and waiting waves do not crowd the memory channels that the last
+      // working tiles need.  The wait is bounded in time (kBarrierTicks of the 100 MHz
+      // wall clock), not in iterations, whose length depends on that contention.
+      wait += 1;
+      u32 rel = b;
+      if ((wait & (kPollEvery - 1)) == 1) {
         u32 r = 0;
-        if (c.lane == 0) r = dev::coherent_load(rel);
-        r = c.bcast(r, 0);
-        if (r >= b + 1 && c.lane == 0) s_released = b + 1;
+        if (c.lane == 0) r = dev::coherent_load(m.barrier + (kMaxBarrierGroups + 1 + g) * kLineWords);
+        rel = (u32)__builtin_amdgcn_readfirstlane(c.bcast(r, 0));
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      const u32 rel = c.bcast(*((volatile u32*)&s_released), 0);
       if (rel > b) {
-        b += 1;
+        b = (u32)__builtin_amdgcn_readfirstlane(b + 1);
         marked = false;
+        wait = 0;
         if (b < m.num_batches) {
-          i = min(m.offsets[b], a.n) + gtile;
           end = min(m.offsets[b + 1], a.n);
+          i = min(m.offsets[b], a.n) + gtile;
           have = i < end;
           if (have) start();
         }
         progressed = true;
-      } else if (++idle > kBarrierGuard) {  // a block never arrived: give up, do not hang
-        c.set_error(kErrStepLimit);
-        break;
+      } else if ((wait & 0xFFFu) == 0) {
+        const u64 now = (u64)wall_clock64();
+        if (wait == 0x1000u) wait_t0 = now;
+        else if (now - wait_t0 > kBarrierTicks) {  // a tile never arrived: give up
+          c.set_error(kErrStepLimit);
+          break;
+        }
       }
     }
     if (__ballot(!progressed) == __ballot(true)) __builtin_amdgcn_s_sleep(1);

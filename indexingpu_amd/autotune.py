@@ -6,7 +6,8 @@ The knobs (all switchable between calls on one index, same tree state):
   * find tile: lanes per lookup of read-only find batches (4/8/16: each lane holds
     16 / width slots of a node; 0 = the tile width);
   * find depth: lookups each tile keeps in flight in read-only find batches (1/2/4/8);
-  * grid share: percent of the resident-block capacity the batched form launches.
+  * grid share: percent of the resident-block capacity the batched form launches;
+  * batched-form mode: one persistent launch with grid barriers, or one launch per batch.
 
 Node fanout and slab size class are not knobs here: every node is one 128-byte slab of
 16 eight-byte slots (border nodes hold up to 14 keys, interior nodes up to 15 children),
@@ -98,6 +99,8 @@ class Autotuner:
             self.index.set_find_depth(cfg["find_depth"])
         if "grid_percent" in cfg:
             self.index.set_multi_grid_percent(cfg["grid_percent"])
+        if "multi_mode" in cfg:
+            self.index.set_multi_mode(cfg["multi_mode"])
 
     def apply(self, op, max_key_length):
         """Applies the cached choice for (op, key-length class); False if none."""

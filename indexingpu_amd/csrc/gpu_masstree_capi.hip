@@ -311,6 +311,13 @@ int gmt_set_find_depth(gmt_index* index, uint32_t depth) {
   return GMT_OK;
 }
 
+int gmt_set_multi_mode(gmt_index* index, uint32_t mode) {
+  if (!index) return bad("gmt_set_multi_mode: null handle");
+  if (mode > 1) return bad("gmt_set_multi_mode: mode must be 0 or 1");
+  B(index)->on_all([&](auto& t) { t.set_multi_mode(mode); });
+  return GMT_OK;
+}
+
 int gmt_set_find_tile(gmt_index* index, uint32_t find_tile) {
   if (!index) return bad("gmt_set_find_tile: null handle");
   if (find_tile != 0 && find_tile != 4 && find_tile != 8 && find_tile != 16)

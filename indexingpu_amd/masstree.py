@@ -70,6 +70,7 @@ def load_library():
     lib.gmt_set_find_depth.argtypes = [vp, u32]
     lib.gmt_multi_batch.argtypes = [vp, P, P, u32, P, P, u32, P, P, P, u32, u64, vp]
     lib.gmt_set_multi_grid_percent.argtypes = [vp, u32]
+    lib.gmt_set_multi_mode.argtypes = [vp, u32]
     lib.gmt_set_tile_size.argtypes = [vp, u32]
     lib.gmt_set_find_tile.argtypes = [vp, u32]
     lib.gmt_tile_size.argtypes = [vp]
@@ -251,6 +252,10 @@ class GpuMasstree:
             _ptr(value_lengths, 4, "value_lengths", n), _ptr(results, 1, "results", n),
             _ptr(offsets, 8, "offsets", int(num_batches) + 1), num_batches, n,
             _stream(stream)))
+
+    def set_multi_mode(self, mode):
+        """Batched form: 0 = one persistent launch, 1 = one launch per batch."""
+        self._check(self._lib.gmt_set_multi_mode(self._h, int(mode)))
 
     def set_multi_grid_percent(self, p):
         self._check(self._lib.gmt_set_multi_grid_percent(self._h, int(p)))

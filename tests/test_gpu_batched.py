@@ -33,14 +33,15 @@ def _run_multi(g, types, keys, lens, vals, offs, mk, mv):
     return out
 
 
-@pytest.mark.parametrize("tile", [16, 64])
-def test_uneven_batches_match_sequential_oracle(tile):
+@pytest.mark.parametrize("tile,mode", [(16, 0), (64, 0), (16, 1), (32, 1)])
+def test_uneven_batches_match_sequential_oracle(tile, mode):
     wl = W.MixedWorkload(50_000, key_len=4, seed=31)
     mk, mv = 4, 2
     rng = np.random.default_rng(5)
     sizes = np.exp(rng.uniform(np.log(16), np.log(20_000), size=40)).astype(int)
     batches = [wl.batch(int(s)) for s in sizes]
     g, o = S.GpuBackend(tile, pool_bytes=128 << 20), S.OracleBackend()
+    g.t.set_multi_mode(mode)
     g.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
     o.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
     types, keys, lens, vals, offs = _concat(batches, mk, mv)
@@ -58,7 +59,8 @@ def test_uneven_batches_match_sequential_oracle(tile):
     assert g.items() == o.items()
 
 
-def test_batches_see_each_other_in_order():
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batches_see_each_other_in_order(mode):
     """insert K / find K / update K / find K / erase K / find K as six batches."""
     rng = np.random.default_rng(9)
     keys, lens = W.unique_random_keys(rng, 3000, 1, 5)
@@ -72,6 +74,7 @@ def test_batches_see_each_other_in_order():
                for t, v in seq]
     types, k, l, vals, offs = _concat(batches, mk, mv)
     g = S.GpuBackend(16, pool_bytes=64 << 20)
+    g.t.set_multi_mode(mode)
     gv, gl, gr = _run_multi(g, types, k, l, vals, offs, mk, mv)
     gv = gv.reshape(6, n, mv)
     gl = gl.reshape(6, n)
