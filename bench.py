@@ -533,7 +533,8 @@ def main():
     log(f"[bench] insert {n / t_ins / 1e3:.1f} Mops/s; tuning find (find tile x depth)")
     # untimed: the autotuner tries tile width x lookups in flight on this batch and
     # leaves the fastest setting applied
-    tuner = Autotuner(tree, find_tiles=(4, 8, 16), find_depths=(1, 2, 4), trials=1)
+    tuner = Autotuner(tree, find_tiles=(4, 8, 16), find_depths=(1, 2, 4), find_passes=(1, 2),
+                      trials=1)
     best = tuner.tune_find(lambda: tree.find(fkeys, mk, flens, out_v, mv, out_l, n), mk)
     log(f"[bench] find config {best}; warmup")
     for _ in range(args.warmup):
@@ -579,7 +580,11 @@ def main():
     # tuned kernel or the workload differ from the profiled ones
     traffic, traffic_note = None, None
     tpath = os.path.join(ROOT, "profiles", "find_kernel_16M_varlen_traffic.json")
-    kname = f"narrow_find_kernel<{best['find_tile']}, {best['find_depth']}>"
+    if best.get("find_passes", 1) == 2:
+        kname = (f"narrow_descend_kernel<{best['find_tile']}, {best['find_depth']}> + "
+                 f"narrow_record_kernel<{best['find_tile']}>")
+    else:
+        kname = f"narrow_find_kernel<{best['find_tile']}, {best['find_depth']}>"
     if os.path.exists(tpath):
         t = json.load(open(tpath))
         if t["kernel"] == kname and n == (1 << 24) and args.seed == 2:
@@ -606,6 +611,7 @@ def main():
             "keys": n, "find_batch": n, "erase_batch": n_erase, "max_key_length": mk,
             "max_value_length": mv, "tile_size": args.tile,
             "find_tile": best["find_tile"], "find_depth": best["find_depth"],
+            "find_passes": best.get("find_passes", 1),
         },
         "insert_mops": round(n / t_ins / 1e3, 2),
         "erase_mops": round(n_erase / t_er / 1e3, 2),
@@ -617,11 +623,12 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": f"narrow_find_kernel<{best['find_tile']}, {best['find_depth']}>",
+            "kernel": kname,
             "algorithmic_bytes_per_launch": abytes,
             "traffic_source": traffic_note,
         },
-        "find_tuning": [{"find_tile": c["find_tile"], "find_depth": c["find_depth"],
+        "find_tuning": [{"find_passes": c.get("find_passes", 1), "find_tile": c["find_tile"],
+                         "find_depth": c["find_depth"],
                          "mops": round(n / ms / 1e3, 1)} for _, _, c, ms in tuner.report()],
     }
     tree.close()

@@ -149,6 +149,12 @@ int gmt_set_find_depth(gmt_index* index, uint32_t depth);
  * depend on it. */
 int gmt_set_find_tile(gmt_index* index, uint32_t find_tile);
 
+/* Read-only finds with a narrow tile as one kernel (passes = 1) or as a descent pass and
+ * a record pass (2, the default; the first call at a new largest batch size allocates 4
+ * bytes per request of scratch and synchronizes).  A tuning knob with no counterpart in
+ * the reference; results do not depend on it. */
+int gmt_set_find_passes(gmt_index* index, uint32_t passes);
+
 /* Tile width (lanes per operation: 16, 32 or 64) that later calls on this index use.
  * The reference fixes it per instance as the tile_size template argument
  * (include/gpu_masstree.hpp:42-46); here all widths share one node layout and one tree

@@ -148,6 +148,7 @@ struct gpu_masstree {
   gpu_masstree& operator=(const gpu_masstree&) = delete;
   ~gpu_masstree() {
     if (scan_stack_) (void)hipFree(scan_stack_);  // per instance, never shared
+    if (park_) (void)hipFree(park_);
     if (owner_) {
       if (err_) (void)hipFree(err_);
       if (barrier_) (void)hipFree(barrier_);
@@ -186,6 +187,9 @@ struct gpu_masstree {
   // 16 / width slots of a node), or 0 for the index's own tile width.
   void set_find_tile(uint32_t t) { find_tile_ = (t == 4 || t == 8 || t == 16) ? t : 0; }
   uint32_t find_tile() const { return find_tile_; }
+  // Narrow finds in one kernel (1) or as a descent pass and a record pass (2).
+  void set_find_passes(uint32_t p) { find_passes_ = p == 1 ? 1u : 2u; }
+  uint32_t find_passes() const { return find_passes_; }
 
   template <bool update_if_exists = false, bool enable_suffix = true, bool use_shmem_key = false>
   void insert(const key_slice_type* keys, const size_type max_key_length,
@@ -603,6 +607,27 @@ struct gpu_masstree {
   template <uint32_t T>
   void launch_narrow(const indexingpu_amd::batch_args& a, hipStream_t stream) {
     constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / T;
+    if (find_passes_ == 2 && a.n > 0) {
+      ensure_park(a.n);
+      auto launch_descend = [&](auto kernel, uint32_t per_block) {
+        const uint64_t want = (a.n + per_block - 1) / per_block;
+        const uint32_t blocks =
+            (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(kernel));
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
+                           view(), a, park_);
+      };
+      switch (find_depth_) {
+        case 1: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 1>, tiles); break;
+        case 4: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 4>, tiles * 4); break;
+        default: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 2>, tiles * 2); break;
+      }
+      auto rec = indexingpu_amd::narrow_record_kernel<T>;
+      const uint64_t want = (a.n + tiles - 1) / tiles;
+      const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(rec));
+      hipLaunchKernelGGL(rec, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream, view(),
+                         a, static_cast<const uint32_t*>(park_));
+      return;
+    }
     switch (find_depth_) {
       case 1: launch_reader(indexingpu_amd::narrow_find_kernel<T, 1>, tiles, a, stream); break;
       case 2: launch_reader(indexingpu_amd::narrow_find_kernel<T, 2>, tiles * 2, a, stream); break;
@@ -618,6 +643,15 @@ struct gpu_masstree {
       case 8: launch_reader(indexingpu_amd::find_kernel<tile_size, 8>, tiles * 8, a, stream); break;
       default: launch_reader(indexingpu_amd::find_kernel<tile_size, 4>, tiles * 4, a, stream); break;
     }
+  }
+
+  // per-lookup scratch of the two-pass find (grows on demand; allocating synchronizes)
+  void ensure_park(uint64_t n) {
+    if (n <= park_words_) return;
+    IGPU_HIP_TRY(hipDeviceSynchronize());
+    if (park_) IGPU_HIP_TRY(hipFree(park_));
+    IGPU_HIP_TRY(hipMalloc(&park_, n * 4));
+    park_words_ = n;
   }
 
   void ensure_scan_stack(uint64_t words) {
@@ -646,6 +680,9 @@ struct gpu_masstree {
   bool owner_ = true;
   uint32_t find_depth_ = 2;
   uint32_t find_tile_ = 8;
+  uint32_t find_passes_ = 2;
+  uint32_t* park_ = nullptr;
+  uint64_t park_words_ = 0;
   uint32_t multi_grid_percent_ = 100;
   uint32_t multi_mode_ = 1;
   uint32_t* scan_stack_ = nullptr;

@@ -279,4 +279,214 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_find_kernel(tree_view tv
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Two-pass read-only find.  Within a wave, the tiles' lookups are at different depths,
+// so one combined kernel executes the interior, border and record code paths on nearly
+// every step.  Here pass 1 (narrow_descend_kernel) runs one uniform step for interior
+// and border nodes -- both the "bounds below the slice" count and the border match are
+// computed, one of them is used -- through the trie layers, and parks each lookup that
+// ends at a record: out_value_lengths[i] = kPendingRecord | record index, and
+// park[i] = keystate << 16 | layer.  Pass 2 (narrow_record_kernel) reads every parked
+// record in one uniform sweep.  Results equal the one-pass kernel's; values of absent
+// keys are not written by either pass.
+constexpr u32 kPendingRecord = 0x80000000u;  // record indices stay below 2^31
+
+template <u32 T, u32 K>
+__global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view tv, batch_args a,
+                                                                        u32* park) {
+  using tile = narrow_tile<T>;
+  constexpr u32 S = tile::S;
+  const tile c;
+  const u64* pool = tv.pool.pool;
+  const u32 nslabs = tv.pool.num_slabs;
+  const u32 tiles_per_block = kBlockThreads / T;
+  const u64 stride = (u64)gridDim.x * tiles_per_block * K;
+  const u64 first = ((u64)blockIdx.x * tiles_per_block + threadIdx.x / T) * K;
+  const u32 maxK = a.max_key_length, maxV = a.max_value_length;
+  u64 idx[K];
+  u32 node[K], layer[K], klen[K], q[K], kc[K];
+  bool active[K];
+
+  auto set_error = [&](u32 code) {
+    if (c.j == 0) atomicCAS(tv.err, 0u, code);
+  };
+  auto slice = [&](u32 k, u32 l) -> u32 {
+    if (l >= T) return a.keys[idx[k] * (u64)maxK + l];
+    return c.from(l, kc[k]);
+  };
+  auto start = [&](u32 k) {
+    active[k] = false;
+    while (idx[k] < a.n) {
+      const u64 i = idx[k];
+      const u32 L = a.key_lengths ? a.key_lengths[i] : maxK;
+      if (L == 0 || L > maxK) {
+        set_error(kErrBadLength);
+        if (c.j == 0) a.out_value_lengths[i] = 0;
+        idx[k] += stride;
+        continue;
+      }
+      klen[k] = L;
+      kc[k] = c.j < L ? a.keys[i * (u64)maxK + c.j] : 0u;
+      node[k] = tv.root;
+      layer[k] = 0;
+      active[k] = true;
+      q[k] = slice(k, 0);
+      return;
+    }
+  };
+  auto finish = [&](u32 k, u32 len) {
+    if (c.j == 0) a.out_value_lengths[idx[k]] = len;
+    idx[k] += stride;
+    start(k);
+  };
+
+#pragma unroll
+  for (u32 k = 0; k < K; k++) {
+    idx[k] = first + k;
+    start(k);
+  }
+  for (u32 guard = 0; guard < (1u << 24); guard++) {
+    bool any = false;
+#pragma unroll
+    for (u32 k = 0; k < K; k++) any |= active[k];
+    if (!any) break;
+    node_piece<S> n[K];
+#pragma unroll
+    for (u32 k = 0; k < K; k++)
+      if (active[k] && node[k] < nslabs) n[k] = load_piece<T>(pool, node[k], c.j);
+#pragma unroll
+    for (u32 k = 0; k < K; k++) {
+      if (!active[k]) continue;
+      if (node[k] >= nslabs) {
+        set_error(kErrBadIndex);
+        finish(k, 0);
+        continue;
+      }
+      const u32 l = layer[k];
+      const u32 h = c.from(T - 1, slot_lo(n[k].s[S - 1]));    // header (slot 15)
+      const u32 sib = c.from(T - 1, slot_hi(n[k].s[S - 1]));  // right sibling
+      const u32 high = c.from(kHighSlot / S, slot_lo(n[k].s[kHighSlot % S]));
+      const u32 ksb = c.from(kHighSlot / S, slot_hi(n[k].s[kHighSlot % S]));
+      const u32 nk = hdr_nkeys(h);
+      const bool border = hdr_border(h);
+      const bool more = l + 1 < klen[k];
+      // interior: child slot = bounds below q; border: the entry equal to (q, more)
+      u32 below = 0, pos = kInvalid;
+#pragma unroll
+      for (u32 t = 0; t < S; t++) {
+        const u32 slot = S * c.j + t;
+        const u32 key = slot_lo(n[k].s[t]);
+        const bool valid = slot < nk;
+        below += c.count(valid && q[k] > key);
+        const u32 f = c.first(valid && key == q[k] && ks_more(ks_get(ksb, slot)) == more);
+        if (f < T) pos = S * f + t;
+      }
+      const bool right = border ? (q[k] > high && sib != kInvalid) : below >= nk;
+      const u32 sel = border ? (pos < kSlots ? pos : 0u) : (below < kSlots ? below : 0u);
+      const u32 v = c.from(sel / S, slot_hi(piece_at<S>(n[k], sel % S)));
+      if (right) {
+        node[k] = sib;
+        if (sib == kInvalid) {
+          set_error(kErrBadIndex);
+          finish(k, 0);
+        }
+        continue;
+      }
+      if (!border) {
+        node[k] = v & kChildMask;
+        continue;
+      }
+      if (pos == kInvalid) {
+        finish(k, 0);
+        continue;
+      }
+      const u32 ks = ks_get(ksb, pos);
+      if (ks == kLink) {
+        layer[k] = l + 1;
+        node[k] = v;
+        q[k] = slice(k, l + 1);
+      } else if (ks == kValue) {
+        if (c.j == 0 && maxV >= 1) a.out_values[idx[k] * (u64)maxV] = v;
+        finish(k, 1);
+      } else if (v >= nslabs) {
+        set_error(kErrBadIndex);
+        finish(k, 0);
+      } else {
+        if (c.j == 0) park[idx[k]] = (ks << 16) | l;
+        finish(k, kPendingRecord | v);
+      }
+    }
+  }
+}
+
+template <u32 T>
+__global__ void __launch_bounds__(kBlockThreads) narrow_record_kernel(tree_view tv, batch_args a,
+                                                                       const u32* park) {
+  using tile = narrow_tile<T>;
+  constexpr u32 S = tile::S;
+  const tile c;
+  const u64* pool = tv.pool.pool;
+  const u32 nslabs = tv.pool.num_slabs;
+  const u32 tiles_per_block = kBlockThreads / T;
+  const u64 ntiles = (u64)gridDim.x * tiles_per_block;
+  const u32 maxK = a.max_key_length, maxV = a.max_value_length;
+  for (u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T; i < a.n; i += ntiles) {
+    const u32 w = a.out_value_lengths[i];
+    if (!(w & kPendingRecord)) continue;
+    const u32 rec = w & ~kPendingRecord;
+    const u32 pk = park[i];
+    const u32 rkind = pk >> 16, l = pk & 0xFFFFu;
+    const u32 klen = a.key_lengths ? a.key_lengths[i] : maxK;
+    node_piece<S> cur = load_piece<T>(pool, rec, c.j);
+    const u32 lw = c.from(0, slot_lo(cur.s[0]));
+    const u32 tl = rec_tail_len(lw), vl = rec_value_len(lw);
+    const u32 total = 1 + tl + vl;
+    const u32 vcopy = vl < maxV ? vl : maxV;
+    const u32* key = a.keys + i * (u64)maxK;
+    u32* out = a.out_values + i * (u64)maxV;
+    bool hit = !(rkind == kSuffix && tl != klen - l - 1);
+    for (u32 d0 = 0; hit;) {
+      // tail words precede value words: every tail word up to a slab's value words has
+      // been compared when they are copied
+      bool mis = false;
+      if (rkind == kSuffix) {
+#pragma unroll
+        for (u32 t = 0; t < S; t++) {
+#pragma unroll
+          for (u32 hh = 0; hh < 2; hh++) {
+            const u32 wd = 2 * (S * c.j + t) + hh;
+            const u32 d = d0 + wd;
+            const u32 v = hh ? slot_hi(cur.s[t]) : slot_lo(cur.s[t]);
+            if (wd < kRecordWordsPerSlab && d >= 1 && d <= tl) mis |= v != key[l + d];
+          }
+        }
+      }
+      if (c.bits(mis)) {
+        hit = false;
+        break;
+      }
+#pragma unroll
+      for (u32 t = 0; t < S; t++) {
+#pragma unroll
+        for (u32 hh = 0; hh < 2; hh++) {
+          const u32 wd = 2 * (S * c.j + t) + hh;
+          const u32 d = d0 + wd;
+          const u32 v = hh ? slot_hi(cur.s[t]) : slot_lo(cur.s[t]);
+          if (wd < kRecordWordsPerSlab && d >= 1 + tl && d < 1 + tl + vcopy) out[d - 1 - tl] = v;
+        }
+      }
+      d0 += kRecordWordsPerSlab;
+      if (d0 >= total) break;
+      const u32 next = c.from(T - 1, slot_hi(cur.s[S - 1]));  // word 31
+      if (next >= nslabs) {
+        if (c.j == 0) atomicCAS(tv.err, 0u, (u32)kErrBadIndex);
+        hit = false;
+        break;
+      }
+      cur = load_piece<T>(pool, next, c.j);
+    }
+    if (c.j == 0) a.out_value_lengths[i] = hit ? vl : 0u;
+  }
+}
+
 }  // namespace indexingpu_amd

@@ -6,6 +6,7 @@ The knobs (all switchable between calls on one index, same tree state):
   * find tile: lanes per lookup of read-only find batches (4/8/16: each lane holds
     16 / width slots of a node; 0 = the tile width);
   * find depth: lookups each tile keeps in flight in read-only find batches (1/2/4/8);
+  * find passes: read-only finds as one kernel or as a descent and a record pass;
   * grid share: percent of the resident-block capacity the batched form launches;
   * batched-form mode: one persistent launch with grid barriers, or one launch per batch.
 
@@ -32,6 +33,7 @@ import time
 TILES = (16, 32, 64)
 FIND_TILES = (4, 8, 16)
 FIND_DEPTHS = (1, 2, 4, 8)
+FIND_PASSES = (1, 2)
 GRID_PERCENTS = (100, 50, 25)
 
 
@@ -78,8 +80,9 @@ class Autotuner:
     """
 
     def __init__(self, index, tiles=TILES, find_tiles=FIND_TILES, find_depths=FIND_DEPTHS,
-                 grid_percents=GRID_PERCENTS, trials=3, timer=None):
+                 grid_percents=GRID_PERCENTS, trials=3, timer=None, find_passes=FIND_PASSES):
         self.index = index
+        self.find_passes = tuple(find_passes)
         self.tiles = tuple(tiles)
         self.find_tiles = tuple(find_tiles)
         self.find_depths = tuple(find_depths)
@@ -95,6 +98,8 @@ class Autotuner:
             self.index.tile_size = cfg["tile"]
         if "find_tile" in cfg:
             self.index.set_find_tile(cfg["find_tile"])
+        if "find_passes" in cfg:
+            self.index.set_find_passes(cfg["find_passes"])
         if "find_depth" in cfg:
             self.index.set_find_depth(cfg["find_depth"])
         if "grid_percent" in cfg:
@@ -123,11 +128,12 @@ class Autotuner:
 
     def tune_find(self, run_find, max_key_length, warmup=1):
         """run_find() launches one read-only find batch (the same batch every call).
-        Every (find tile, depth) candidate is warmed up, then timed `trials` times; the
+        Every (passes, find tile, depth) candidate is warmed up, then timed `trials` times; the
         minimum counts."""
         scored = []
-        for tile, depth in itertools.product(self.find_tiles, self.find_depths):
-            cfg = {"find_tile": tile, "find_depth": depth}
+        for passes, tile, depth in itertools.product(self.find_passes, self.find_tiles,
+                                                     self.find_depths):
+            cfg = {"find_passes": passes, "find_tile": tile, "find_depth": depth}
             self.apply_config(cfg)
             for _ in range(warmup):
                 self.timer(run_find)

@@ -311,6 +311,13 @@ int gmt_set_find_depth(gmt_index* index, uint32_t depth) {
   return GMT_OK;
 }
 
+int gmt_set_find_passes(gmt_index* index, uint32_t passes) {
+  if (!index) return bad("gmt_set_find_passes: null handle");
+  if (passes != 1 && passes != 2) return bad("gmt_set_find_passes: passes must be 1 or 2");
+  B(index)->on_all([&](auto& t) { t.set_find_passes(passes); });
+  return GMT_OK;
+}
+
 int gmt_set_multi_mode(gmt_index* index, uint32_t mode) {
   if (!index) return bad("gmt_set_multi_mode: null handle");
   if (mode > 1) return bad("gmt_set_multi_mode: mode must be 0 or 1");

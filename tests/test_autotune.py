@@ -9,10 +9,14 @@ class FakeIndex:
         self.depth = 4
         self.grid = 100
         self.ftile = 8
+        self.passes = 2
         self.log = []
 
     def set_find_tile(self, t):
         self.ftile = t
+
+    def set_find_passes(self, p):
+        self.passes = p
 
     def set_find_depth(self, k):
         self.depth = k
@@ -32,7 +36,7 @@ class FakeClock:
         fn()
         self.calls += 1
         i = self.index
-        return self.cost(i.tile_size, i.depth, i.grid, i.ftile)
+        return self.cost(i.tile_size, i.depth, i.grid, i.ftile) + (0.5 if i.passes == 1 else 0.0)
 
 
 def test_length_class():
@@ -47,13 +51,13 @@ def test_tune_find_picks_minimum_and_applies_it():
     tu = Autotuner(idx, trials=2, timer=clock)
     runs = []
     cfg = tu.tune_find(lambda: runs.append((idx.ftile, idx.depth)), max_key_length=8)
-    assert cfg == {"find_tile": 8, "find_depth": 2}
-    assert (idx.ftile, idx.depth) == (8, 2)
-    # 3 tiles x 4 depths, each 1 warmup + 2 timed runs
-    assert len(runs) == 12 * 3 and clock.calls == 12 * 3
+    assert cfg == {"find_passes": 2, "find_tile": 8, "find_depth": 2}
+    assert (idx.passes, idx.ftile, idx.depth) == (2, 8, 2)
+    # 2 pass counts x 3 tiles x 4 depths, each 1 warmup + 2 timed runs
+    assert len(runs) == 24 * 3 and clock.calls == 24 * 3
     assert tu.best("find", 7) == cfg          # same length class (8)
     assert tu.best("find", 9) is None          # class 16 not tuned
-    assert len(tu.report()) == 12
+    assert len(tu.report()) == 24
 
 
 def test_tune_online_scores_per_operation_and_never_reuses_units():
@@ -81,8 +85,8 @@ def test_apply_restores_cached_choice():
     idx = FakeIndex()
     tu = Autotuner(idx, timer=FakeClock(idx, lambda t, d, g, f: f + d))
     tu.tune_find(lambda: None, max_key_length=4)
-    assert (idx.ftile, idx.depth) == (4, 1)
-    idx.ftile, idx.depth = 16, 8
+    assert (idx.passes, idx.ftile, idx.depth) == (2, 4, 1)
+    idx.passes, idx.ftile, idx.depth = 1, 16, 8
     assert tu.apply("find", 3)
-    assert (idx.ftile, idx.depth) == (4, 1)
+    assert (idx.passes, idx.ftile, idx.depth) == (2, 4, 1)
     assert not tu.apply("mixed", 3)

@@ -1,5 +1,5 @@
-"""Read-only finds at every find tile width (4, 8, 16 lanes, and 0 = the index's tile)
-and every in-flight depth against the oracle: keys of 1..40 slices sharing prefixes
+"""Read-only finds at every find tile width (4, 8, 16 lanes, and 0 = the index's tile),
+every in-flight depth and both pass counts against the oracle: keys of 1..40 slices sharing prefixes
 (trie layers, suffix records whose tails span slabs), values of 0..70 slices (inline
 values, records of several slabs), hits and misses, truncation by max_value_length."""
 import numpy as np
@@ -48,10 +48,12 @@ def loaded():
 @pytest.mark.parametrize("find_tile", [0, 4, 8, 16])
 @pytest.mark.parametrize("depth", [1, 2, 4, 8])
 @pytest.mark.parametrize("max_value", [70, 3])
-def test_find_tiles_match_oracle(loaded, find_tile, depth, max_value):
+@pytest.mark.parametrize("passes", [1, 2])
+def test_find_tiles_match_oracle(loaded, find_tile, depth, max_value, passes):
     g, o, keys, lens, mk = loaded
     g.t.set_find_tile(find_tile)
     g.t.set_find_depth(depth)
+    g.t.set_find_passes(passes)
     got_v, got_l = g.find(keys, lens, mk, max_value)
     exp_v, exp_l = o.find(keys, lens, mk, max_value)
     assert np.array_equal(got_l, exp_l)
