@@ -20,7 +20,7 @@ namespace indexingpu_amd {
 
 template <u32 T>
 struct narrow_tile {
-  static_assert(T == 4 || T == 8 || T == 16, "narrow tiles are 4, 8 or 16 lanes");
+  static_assert(T == 2 || T == 4 || T == 8 || T == 16, "narrow tiles are 2, 4, 8 or 16 lanes");
   static constexpr u32 S = kSlots / T;  // slots per lane
   static constexpr u32 U = 1;           // key slices cached per lane (T per tile)
   u32 j;       // lane within the tile
