@@ -533,7 +533,7 @@ def main():
     log(f"[bench] insert {n / t_ins / 1e3:.1f} Mops/s; tuning find (find tile x depth)")
     # untimed: the autotuner tries tile width x lookups in flight on this batch and
     # leaves the fastest setting applied
-    tuner = Autotuner(tree, find_tiles=(2, 4, 8, 16), find_depths=(1, 2, 4), find_passes=(1, 2),
+    tuner = Autotuner(tree, find_tiles=(1, 2, 4, 8, 16), find_depths=(1, 2, 4), find_passes=(1, 2),
                       trials=1)
     best = tuner.tune_find(lambda: tree.find(fkeys, mk, flens, out_v, mv, out_l, n), mk)
     log(f"[bench] find config {best}; warmup")

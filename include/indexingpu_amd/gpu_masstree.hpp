@@ -172,6 +172,7 @@ struct gpu_masstree {
       launch_batch<true>(a, stream, false);
     } else {
       switch (find_tile_) {
+        case 1: launch_narrow<1>(a, stream); break;
         case 2: launch_narrow<2>(a, stream); break;
         case 4: launch_narrow<4>(a, stream); break;
         case 8: launch_narrow<8>(a, stream); break;
@@ -186,7 +187,9 @@ struct gpu_masstree {
   uint32_t find_depth() const { return find_depth_; }
   // Lanes per lookup of read-only finds: 4, 8 or 16 (narrow tiles, each lane holding
   // 16 / width slots of a node), or 0 for the index's own tile width.
-  void set_find_tile(uint32_t t) { find_tile_ = (t == 2 || t == 4 || t == 8 || t == 16) ? t : 0; }
+  void set_find_tile(uint32_t t) {
+    find_tile_ = (t == 1 || t == 2 || t == 4 || t == 8 || t == 16) ? t : 0;
+  }
   uint32_t find_tile() const { return find_tile_; }
   // Narrow finds in one kernel (1) or as a descent pass and a record pass (2).
   void set_find_passes(uint32_t p) { find_passes_ = p == 1 ? 1u : 2u; }

@@ -1,8 +1,8 @@
 // Read-only find with narrow tiles (the find<concurrent=false> batch path).
 //
-// A tile of T lanes (4, 8 or 16) reads a 128-byte node as T pieces of 128/T bytes: lane
-// j holds slots S*j .. S*j + S - 1 (S = 16 / T), fetched with one dwordx2 (T = 16), one
-// dwordx4 (T = 8) or two dwordx4 (T = 4) loads.  A 64-lane wave therefore advances
+// A tile of T lanes (1, 2, 4, 8 or 16) reads a 128-byte node as T pieces of 128/T bytes:
+// lane j holds slots S*j .. S*j + S - 1 (S = 16 / T), fetched with one dwordx2 (T = 16)
+// or 16/(2T) dwordx4 loads (T <= 8).  A 64-lane wave therefore advances
 // 64/T lookups per node step, and each lookup keeps K of its own in flight, so a wave
 // has 64K/T independent node loads outstanding.  Per node step a tile needs:
 //   * the header and sibling words (slot 15) from its last lane (ds_bpermute);
@@ -20,7 +20,8 @@ namespace indexingpu_amd {
 
 template <u32 T>
 struct narrow_tile {
-  static_assert(T == 2 || T == 4 || T == 8 || T == 16, "narrow tiles are 2, 4, 8 or 16 lanes");
+  static_assert(T == 1 || T == 2 || T == 4 || T == 8 || T == 16,
+                "narrow tiles are 1, 2, 4, 8 or 16 lanes");
   static constexpr u32 S = kSlots / T;  // slots per lane
   static constexpr u32 U = 1;           // key slices cached per lane (T per tile)
   u32 j;       // lane within the tile
@@ -33,16 +34,31 @@ struct narrow_tile {
     base = l & ~(T - 1);
     tmask = ((1ull << T) - 1) << base;
   }
-  // value v of lane src of this tile
+  // value v of lane src of this tile (a one-lane tile holds everything itself)
   __device__ __forceinline__ u32 from(u32 src, u32 v) const {
-    return (u32)__builtin_amdgcn_ds_bpermute((int)((base + src) << 2), (int)v);
+    if constexpr (T == 1) {
+      (void)src;
+      return v;
+    } else {
+      return (u32)__builtin_amdgcn_ds_bpermute((int)((base + src) << 2), (int)v);
+    }
   }
-  __device__ __forceinline__ u64 bits(bool p) const { return __ballot(p) & tmask; }
-  __device__ __forceinline__ u32 count(bool p) const { return (u32)__popcll(bits(p)); }
+  __device__ __forceinline__ u64 bits(bool p) const {
+    if constexpr (T == 1) return p ? tmask : 0ull;
+    else return __ballot(p) & tmask;
+  }
+  __device__ __forceinline__ u32 count(bool p) const {
+    if constexpr (T == 1) return p ? 1u : 0u;
+    else return (u32)__popcll(bits(p));
+  }
   // lane index (within the tile) of the first lane with p, or T
   __device__ __forceinline__ u32 first(bool p) const {
-    const u64 m = bits(p);
-    return m ? (u32)(__ffsll((unsigned long long)m) - 1) - base : T;
+    if constexpr (T == 1) {
+      return p ? 0u : 1u;
+    } else {
+      const u64 m = bits(p);
+      return m ? (u32)(__ffsll((unsigned long long)m) - 1) - base : T;
+    }
   }
 };
 

@@ -31,7 +31,7 @@ import itertools
 import time
 
 TILES = (16, 32, 64)
-FIND_TILES = (2, 4, 8, 16)
+FIND_TILES = (1, 2, 4, 8, 16)
 FIND_DEPTHS = (1, 2, 4, 8)
 FIND_PASSES = (1, 2)
 GRID_PERCENTS = (100, 50, 25)

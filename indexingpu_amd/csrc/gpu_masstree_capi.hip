@@ -327,8 +327,8 @@ int gmt_set_multi_mode(gmt_index* index, uint32_t mode) {
 
 int gmt_set_find_tile(gmt_index* index, uint32_t find_tile) {
   if (!index) return bad("gmt_set_find_tile: null handle");
-  if (find_tile != 0 && find_tile != 2 && find_tile != 4 && find_tile != 8 && find_tile != 16)
-    return bad("gmt_set_find_tile: find_tile must be 0, 2, 4, 8 or 16");
+  if (find_tile > 16 || (find_tile & (find_tile - 1)) != 0)
+    return bad("gmt_set_find_tile: find_tile must be 0, 1, 2, 4, 8 or 16");
   B(index)->on_all([&](auto& t) { t.set_find_tile(find_tile); });
   return GMT_OK;
 }

@@ -53,11 +53,11 @@ def test_tune_find_picks_minimum_and_applies_it():
     cfg = tu.tune_find(lambda: runs.append((idx.ftile, idx.depth)), max_key_length=8)
     assert cfg == {"find_passes": 2, "find_tile": 8, "find_depth": 2}
     assert (idx.passes, idx.ftile, idx.depth) == (2, 8, 2)
-    # 2 pass counts x 4 tiles x 4 depths, each 1 warmup + 2 timed runs
-    assert len(runs) == 32 * 3 and clock.calls == 32 * 3
+    # 2 pass counts x 5 tiles x 4 depths, each 1 warmup + 2 timed runs
+    assert len(runs) == 40 * 3 and clock.calls == 40 * 3
     assert tu.best("find", 7) == cfg          # same length class (8)
     assert tu.best("find", 9) is None          # class 16 not tuned
-    assert len(tu.report()) == 32
+    assert len(tu.report()) == 40
 
 
 def test_tune_online_scores_per_operation_and_never_reuses_units():
@@ -85,8 +85,8 @@ def test_apply_restores_cached_choice():
     idx = FakeIndex()
     tu = Autotuner(idx, timer=FakeClock(idx, lambda t, d, g, f: f + d))
     tu.tune_find(lambda: None, max_key_length=4)
-    assert (idx.passes, idx.ftile, idx.depth) == (2, 2, 1)
+    assert (idx.passes, idx.ftile, idx.depth) == (2, 1, 1)
     idx.passes, idx.ftile, idx.depth = 1, 16, 8
     assert tu.apply("find", 3)
-    assert (idx.passes, idx.ftile, idx.depth) == (2, 2, 1)
+    assert (idx.passes, idx.ftile, idx.depth) == (2, 1, 1)
     assert not tu.apply("mixed", 3)

@@ -45,7 +45,7 @@ def loaded():
     g.t.close()
 
 
-@pytest.mark.parametrize("find_tile", [0, 2, 4, 8, 16])
+@pytest.mark.parametrize("find_tile", [0, 1, 2, 4, 8, 16])
 @pytest.mark.parametrize("depth", [1, 2, 4, 8])
 @pytest.mark.parametrize("max_value", [70, 3])
 @pytest.mark.parametrize("passes", [1, 2])
