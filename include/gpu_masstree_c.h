@@ -140,11 +140,11 @@ uint64_t gmt_slabs_allocated(gmt_index* index);
 uint64_t gmt_pool_slabs(gmt_index* index);
 
 /* Lookups each tile keeps in flight in read-only find batches (1, 2, 4 or 8; other
- * values select 4; default 2).  A tuning knob with no counterpart in the reference. */
+ * values select 4; default 1).  A tuning knob with no counterpart in the reference. */
 int gmt_set_find_depth(gmt_index* index, uint32_t depth);
 
 /* Lanes per lookup of read-only find batches (gmt_find with concurrent = 0): 1, 2, 4, 8, 16
- * (each lane holds 16 / find_tile slots of a node; default 8), or 0 for the index's
+ * (each lane holds 16 / find_tile slots of a node; default 2), or 0 for the index's
  * tile width.  A tuning knob with no counterpart in the reference; results do not
  * depend on it. */
 int gmt_set_find_tile(gmt_index* index, uint32_t find_tile);

@@ -682,8 +682,8 @@ struct gpu_masstree {
 
   uint64_t* inserted_hint_ = nullptr;  // host counter shared with attached instances
   bool owner_ = true;
-  uint32_t find_depth_ = 2;
-  uint32_t find_tile_ = 8;
+  uint32_t find_depth_ = 1;
+  uint32_t find_tile_ = 2;
   uint32_t find_passes_ = 2;
   uint32_t* park_ = nullptr;
   uint64_t park_words_ = 0;

@@ -1,4 +1,4 @@
-"""Read-only finds at every find tile width (4, 8, 16 lanes, and 0 = the index's tile),
+"""Read-only finds at every find tile width (1, 2, 4, 8, 16 lanes, and 0 = the index's tile),
 every in-flight depth and both pass counts against the oracle: keys of 1..40 slices sharing prefixes
 (trie layers, suffix records whose tails span slabs), values of 0..70 slices (inline
 values, records of several slabs), hits and misses, truncation by max_value_length."""
