@@ -298,67 +298,131 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_find_kernel(tree_view tv
 // ---------------------------------------------------------------------------------
 // Two-pass read-only find.  Within a wave, the tiles' lookups are at different depths,
 // so one combined kernel executes the interior, border and record code paths on nearly
-// every step.  Here pass 1 (narrow_descend_kernel) runs one uniform step for interior
-// and border nodes -- both the "bounds below the slice" count and the border match are
-// computed, one of them is used -- through the trie layers, and parks each lookup that
-// ends at a record: out_value_lengths[i] = kPendingRecord | record index, and
-// park[i] = keystate << 16 | layer.  Pass 2 (narrow_record_kernel) reads every parked
-// record in one uniform sweep.  Results equal the one-pass kernel's; values of absent
-// keys are not written by either pass.
+// every step.  Here pass 1 (narrow_descend_kernel) walks interior and border nodes
+// through the trie layers and parks each lookup that ends at a record:
+// out_value_lengths[i] = kPendingRecord | record index, and park[i] = keystate << 16 |
+// layer.  Pass 2 (narrow_record_kernel) reads every parked record in one uniform sweep.
+// Results equal the one-pass kernel's; values of absent keys are not written by either
+// pass.
 constexpr u32 kPendingRecord = 0x80000000u;  // record indices stay below 2^31
 
+// Exchanges inside a tile for the descent.  Tiles of two or four lanes use DPP quad
+// permutations (an operand modifier of a VALU instruction: no LDS traffic, no wait);
+// wider tiles use ds_bpermute.  Every lane of a tile takes the same branches, so the
+// lanes a permutation reads are active.
+template <u32 T>
+struct tile_xchg {
+  u32 j;     // lane within the tile
+  u32 base;  // first lane of the tile within the wave
+  __device__ __forceinline__ tile_xchg() {
+    const u32 l = threadIdx.x & 63u;
+    j = l & (T - 1);
+    base = l & ~(T - 1);
+  }
+  // quad_perm control: lane i of each quad reads lane sel(i) of the same quad
+  template <u32 L0, u32 L1, u32 L2, u32 L3>
+  static __device__ __forceinline__ u32 quad(u32 v) {
+    constexpr int ctrl = (int)(L0 | (L1 << 2) | (L2 << 4) | (L3 << 6));
+    return (u32)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xF, 0xF, false);
+  }
+  // v of tile lane Src
+  template <u32 Src>
+  __device__ __forceinline__ u32 lane(u32 v) const {
+    static_assert(Src < T, "source lane outside the tile");
+    if constexpr (T == 1) {
+      return v;
+    } else if constexpr (T == 2) {
+      return quad<Src, Src, 2 + Src, 2 + Src>(v);
+    } else if constexpr (T == 4) {
+      return quad<Src, Src, Src, Src>(v);
+    } else {
+      return (u32)__builtin_amdgcn_ds_bpermute((int)((base + Src) << 2), (int)v);
+    }
+  }
+  // v of the lane whose index differs from this one's in bit M
+  template <u32 M>
+  __device__ __forceinline__ u32 partner(u32 v) const {
+    if constexpr (M == 1) {
+      return quad<1, 0, 3, 2>(v);
+    } else if constexpr (M == 2) {
+      return quad<2, 3, 0, 1>(v);
+    } else {
+      const u32 src = (threadIdx.x & 63u) ^ M;
+      return (u32)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+    }
+  }
+  __device__ __forceinline__ u32 min(u32 v) const {
+    if constexpr (T >= 2) v = umin(v, partner<1>(v));
+    if constexpr (T >= 4) v = umin(v, partner<2>(v));
+    if constexpr (T >= 8) v = umin(v, partner<4>(v));
+    if constexpr (T >= 16) v = umin(v, partner<8>(v));
+    return v;
+  }
+  __device__ __forceinline__ u32 bor(u32 v) const {
+    if constexpr (T >= 2) v |= partner<1>(v);
+    if constexpr (T >= 4) v |= partner<2>(v);
+    if constexpr (T >= 8) v |= partner<4>(v);
+    if constexpr (T >= 16) v |= partner<8>(v);
+    return v;
+  }
+  static __device__ __forceinline__ u32 umin(u32 a, u32 b) { return a < b ? a : b; }
+};
+
+// One descent step per node: every lane finds the lowest slot of its piece that
+// qualifies (interior: the first bound >= the slice; border: the entry equal to
+// (slice, continues?)) with two selects per slot, then the tile takes the minimum slot
+// and the owner's child/value with two exchanges.  Slots at or past the node's key count
+// hold no entry, but every such slot lies above every real entry, so the tile's lowest
+// qualifying slot is a real entry exactly when it is below the key count.
 template <u32 T, u32 K>
 __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view tv, batch_args a,
                                                                         u32* park) {
-  using tile = narrow_tile<T>;
-  constexpr u32 S = tile::S;
-  const tile c;
+  constexpr u32 S = kSlots / T;  // slots per lane
+  constexpr u32 kNone = 32;      // no qualifying slot in this lane's piece
+  const tile_xchg<T> x;
   const u64* pool = tv.pool.pool;
   const u32 nslabs = tv.pool.num_slabs;
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 stride = (u64)gridDim.x * tiles_per_block * K;
   const u64 first = ((u64)blockIdx.x * tiles_per_block + threadIdx.x / T) * K;
   const u32 maxK = a.max_key_length, maxV = a.max_value_length;
+  const u32 slot0 = S * x.j;  // this lane's first slot
+  if (a.n == 0) return;
   u64 idx[K];
-  u32 node[K], layer[K], klen[K], q[K], kc[K];
-  bool active[K];
+  u32 node[K], layer[K], klen[K], q[K];
+  u32 next_len[K], next_q[K];  // length and first slice of lookup idx[k], loaded ahead
+  bool active[K], q_stale[K];
 
-  auto set_error = [&](u32 code) {
-    if (c.j == 0) atomicCAS(tv.err, 0u, code);
+  // loads of the tile's following lookup (the index clamped into the batch, so the loads
+  // are unconditional and land in the registers the next start reads)
+  auto fetch = [&](u32 k, u64 i) {
+    const u64 c = i < a.n ? i : a.n - 1;
+    next_len[k] = a.key_lengths ? a.key_lengths[c] : maxK;
+    next_q[k] = a.keys[c * (u64)maxK];
   };
-  auto slice = [&](u32 k, u32 l) -> u32 {
-    if (l >= T) return a.keys[idx[k] * (u64)maxK + l];
-    return c.from(l, kc[k]);
-  };
+  // begins lookup idx[k] (its length and first slice are in next_len/next_q, loaded one
+  // lookup ahead) and loads those of the tile's following lookup, which arrive while
+  // this one descends.  A length outside 1..max_key_length is reported at the lookup's
+  // first step.
   auto start = [&](u32 k) {
-    active[k] = false;
-    while (idx[k] < a.n) {
-      const u64 i = idx[k];
-      const u32 L = a.key_lengths ? a.key_lengths[i] : maxK;
-      if (L == 0 || L > maxK) {
-        set_error(kErrBadLength);
-        if (c.j == 0) a.out_value_lengths[i] = 0;
-        idx[k] += stride;
-        continue;
-      }
-      klen[k] = L;
-      kc[k] = c.j < L ? a.keys[i * (u64)maxK + c.j] : 0u;
-      node[k] = tv.root;
-      layer[k] = 0;
-      active[k] = true;
-      q[k] = slice(k, 0);
-      return;
-    }
-  };
-  auto finish = [&](u32 k, u32 len) {
-    if (c.j == 0) a.out_value_lengths[idx[k]] = len;
-    idx[k] += stride;
-    start(k);
+    // copy the values into their own registers here, before the following loads are
+    // issued into next_len/next_q (a copy placed after those loads would wait for them)
+    u32 L, q0;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(L) : "v"(next_len[k]));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(q0) : "v"(next_q[k]));
+    klen[k] = L;
+    q[k] = q0;
+    node[k] = tv.root;
+    layer[k] = 0;
+    q_stale[k] = false;
+    active[k] = idx[k] < a.n;
+    fetch(k, idx[k] + stride);
   };
 
 #pragma unroll
   for (u32 k = 0; k < K; k++) {
     idx[k] = first + k;
+    fetch(k, idx[k]);
     start(k);
   }
   for (u32 guard = 0; guard < (1u << 24); guard++) {
@@ -366,70 +430,87 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
 #pragma unroll
     for (u32 k = 0; k < K; k++) any |= active[k];
     if (!any) break;
+    // node[k] is a pool index here (the root after start; a step that leaves an index
+    // outside the pool finishes the lookup), so every tile loads unconditionally.  A
+    // lookup that entered a new trie layer loads that layer's slice beside its node.
     node_piece<S> n[K];
 #pragma unroll
-    for (u32 k = 0; k < K; k++)
-      if (active[k] && node[k] < nslabs) n[k] = load_piece<T>(pool, node[k], c.j);
+    for (u32 k = 0; k < K; k++) {
+      if (layer[k] != 0 && q_stale[k]) {
+        q[k] = a.keys[idx[k] * (u64)maxK + layer[k]];
+        q_stale[k] = false;
+      }
+      n[k] = load_piece<T>(pool, node[k], x.j);
+    }
 #pragma unroll
     for (u32 k = 0; k < K; k++) {
       if (!active[k]) continue;
-      if (node[k] >= nslabs) {
-        set_error(kErrBadIndex);
-        finish(k, 0);
-        continue;
-      }
-      const u32 l = layer[k];
-      const u32 h = c.from(T - 1, slot_lo(n[k].s[S - 1]));    // header (slot 15)
-      const u32 sib = c.from(T - 1, slot_hi(n[k].s[S - 1]));  // right sibling
-      const u32 high = c.from(kHighSlot / S, slot_lo(n[k].s[kHighSlot % S]));
-      const u32 ksb = c.from(kHighSlot / S, slot_hi(n[k].s[kHighSlot % S]));
+      const u32 h = x.template lane<T - 1>(slot_lo(n[k].s[S - 1]));    // header (slot 15)
+      const u32 sib = x.template lane<T - 1>(slot_hi(n[k].s[S - 1]));  // right sibling
       const u32 nk = hdr_nkeys(h);
       const bool border = hdr_border(h);
-      const bool more = l + 1 < klen[k];
-      // interior: child slot = bounds below q; border: the entry equal to (q, more)
-      u32 below = 0, pos = kInvalid;
-#pragma unroll
-      for (u32 t = 0; t < S; t++) {
-        const u32 slot = S * c.j + t;
-        const u32 key = slot_lo(n[k].s[t]);
-        const bool valid = slot < nk;
-        below += c.count(valid && q[k] > key);
-        const u32 f = c.first(valid && key == q[k] && ks_more(ks_get(ksb, slot)) == more);
-        if (f < T) pos = S * f + t;
-      }
-      const bool right = border ? (q[k] > high && sib != kInvalid) : below >= nk;
-      const u32 sel = border ? (pos < kSlots ? pos : 0u) : (below < kSlots ? below : 0u);
-      const u32 v = c.from(sel / S, slot_hi(piece_at<S>(n[k], sel % S)));
-      if (right) {
-        node[k] = sib;
-        if (sib == kInvalid) {
-          set_error(kErrBadIndex);
-          finish(k, 0);
-        }
-        continue;
-      }
+      const u32 l = layer[k];
+      u32 lt = kNone, cand = 0, high = 0, ksb = 0;
       if (!border) {
-        node[k] = v & kChildMask;
-        continue;
-      }
-      if (pos == kInvalid) {
-        finish(k, 0);
-        continue;
-      }
-      const u32 ks = ks_get(ksb, pos);
-      if (ks == kLink) {
-        layer[k] = l + 1;
-        node[k] = v;
-        q[k] = slice(k, l + 1);
-      } else if (ks == kValue) {
-        if (c.j == 0 && maxV >= 1) a.out_values[idx[k] * (u64)maxV] = v;
-        finish(k, 1);
-      } else if (v >= nslabs) {
-        set_error(kErrBadIndex);
-        finish(k, 0);
+#pragma unroll
+        for (int t = S - 1; t >= 0; t--) {
+          const bool c = slot_lo(n[k].s[t]) >= q[k];
+          lt = c ? (u32)t : lt;
+          cand = c ? slot_hi(n[k].s[t]) : cand;
+        }
       } else {
-        if (c.j == 0) park[idx[k]] = (ks << 16) | l;
-        finish(k, kPendingRecord | v);
+        high = x.template lane<kHighSlot / S>(slot_lo(n[k].s[kHighSlot % S]));
+        ksb = x.template lane<kHighSlot / S>(slot_hi(n[k].s[kHighSlot % S]));
+        // the continues-bit of slot s is bit 2s + 1 of the keystates; after the shift,
+        // bit 2t is that of this lane's slot t, inverted when the key ends here
+        u32 P = ksb >> (2 * slot0 + 1);
+        if (!(l + 1 < klen[k])) P = ~P;
+#pragma unroll
+        for (int t = S - 1; t >= 0; t--) {
+          const bool c = slot_lo(n[k].s[t]) == q[k] && ((P >> (2 * t)) & 1u);
+          lt = c ? (u32)t : lt;
+          cand = c ? slot_hi(n[k].s[t]) : cand;
+        }
+      }
+      const u32 mine = slot0 + lt;
+      const u32 gid = x.min(mine);
+      const u32 v = x.bor(mine == gid ? cand : 0u);
+      const bool found = gid < nk;
+      u32 len = kInvalid;  // kInvalid: the lookup goes on
+      if (klen[k] == 0 || klen[k] > maxK) {
+        if (x.j == 0) atomicCAS(tv.err, 0u, (u32)kErrBadLength);
+        len = 0;
+      } else if (!border) {
+        // no bound >= q: q lies right of this node (B-link move right)
+        node[k] = found ? (v & kChildMask) : sib;
+      } else if (q[k] > high && sib != kInvalid) {
+        node[k] = sib;
+      } else if (!found) {
+        len = 0;
+      } else {
+        const u32 ks = ks_get(ksb, gid);
+        if (ks == kLink) {
+          layer[k] = l + 1;  // < key length: the entry continues
+          node[k] = v;
+          q_stale[k] = true;
+        } else if (ks == kValue) {
+          if (x.j == 0 && maxV >= 1) a.out_values[idx[k] * (u64)maxV] = v;
+          len = 1;
+        } else if (v < nslabs) {
+          if (x.j == 0) park[idx[k]] = (ks << 16) | l;
+          len = kPendingRecord | v;
+        } else {
+          node[k] = v;  // reported below
+        }
+      }
+      if (len == kInvalid && node[k] >= nslabs) {
+        if (x.j == 0) atomicCAS(tv.err, 0u, (u32)kErrBadIndex);
+        len = 0;
+      }
+      if (len != kInvalid) {
+        if (x.j == 0) a.out_value_lengths[idx[k]] = len;
+        idx[k] += stride;
+        start(k);
       }
     }
   }
