@@ -485,6 +485,9 @@ def main():
                     help="run one device-to-device copy of this many MiB first (a kernel of "
                          "known traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--find-config", default="",
+                    help="passes,find_tile,depth: pin the read-only find configuration "
+                         "instead of tuning it (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: every box in this pool has one GPU, so "
@@ -533,8 +536,13 @@ def main():
     log(f"[bench] insert {n / t_ins / 1e3:.1f} Mops/s; tuning find (find tile x depth)")
     # untimed: the autotuner tries tile width x lookups in flight on this batch and
     # leaves the fastest setting applied
-    tuner = Autotuner(tree, find_tiles=(1, 2, 4, 8, 16), find_depths=(1, 2, 4), find_passes=(1, 2),
-                      trials=1)
+    if args.find_config:
+        # a pinned configuration (profiling runs): the tuner measures only that one
+        fp, ft, fd = (int(v) for v in args.find_config.split(","))
+        tuner = Autotuner(tree, find_tiles=(ft,), find_depths=(fd,), find_passes=(fp,), trials=1)
+    else:
+        tuner = Autotuner(tree, find_tiles=(1, 2, 4, 8, 16), find_depths=(1, 2, 4),
+                          find_passes=(1, 2), trials=1)
     best = tuner.tune_find(lambda: tree.find(fkeys, mk, flens, out_v, mv, out_l, n), mk)
     log(f"[bench] find config {best}; warmup")
     for _ in range(args.warmup):

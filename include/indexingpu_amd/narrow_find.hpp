@@ -516,73 +516,108 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   }
 }
 
+// Pass 2: every parked lookup reads its record.  A tile takes entries i, i + tiles, ...;
+// the output word, parked state and key length of its next entry are loaded one entry
+// ahead, and the key-tail words a match compares in the record's first slab are loaded
+// beside the record, so a lookup waits for one memory round trip.
 template <u32 T>
 __global__ void __launch_bounds__(kBlockThreads) narrow_record_kernel(tree_view tv, batch_args a,
                                                                        const u32* park) {
-  using tile = narrow_tile<T>;
-  constexpr u32 S = tile::S;
-  const tile c;
+  constexpr u32 S = kSlots / T;  // slots per lane: words 2*S*j .. 2*S*j + 2*S - 1 of a slab
+  constexpr u32 W = 2 * S;
+  const tile_xchg<T> x;
   const u64* pool = tv.pool.pool;
   const u32 nslabs = tv.pool.num_slabs;
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
   const u32 maxK = a.max_key_length, maxV = a.max_value_length;
-  for (u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T; i < a.n; i += ntiles) {
-    const u32 w = a.out_value_lengths[i];
+  const u32 w0 = W * x.j;  // this lane's first word of a slab
+  if (a.n == 0) return;
+  u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
+  u32 next_w, next_pk, next_len;
+  auto fetch = [&](u64 e) {
+    const u64 c = e < a.n ? e : a.n - 1;  // clamped: the loads are unconditional
+    next_w = a.out_value_lengths[c];
+    next_pk = park[c];
+    next_len = a.key_lengths ? a.key_lengths[c] : maxK;
+  };
+  fetch(i);
+  for (; i < a.n; i += ntiles) {
+    // copies made before the next entry's loads are issued into the same registers
+    u32 w, pk, klen;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(w) : "v"(next_w));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(pk) : "v"(next_pk));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(klen) : "v"(next_len));
+    fetch(i + ntiles);
     if (!(w & kPendingRecord)) continue;
-    const u32 rec = w & ~kPendingRecord;
-    const u32 pk = park[i];
+    const u32 rec = w & ~kPendingRecord;  // below the pool size: checked by pass 1
     const u32 rkind = pk >> 16, l = pk & 0xFFFFu;
-    const u32 klen = a.key_lengths ? a.key_lengths[i] : maxK;
-    node_piece<S> cur = load_piece<T>(pool, rec, c.j);
-    const u32 lw = c.from(0, slot_lo(cur.s[0]));
+    const u32* key = a.keys + i * (u64)maxK;
+    u32* out = a.out_values + i * (u64)maxV;
+    node_piece<S> cur = load_piece<T>(pool, rec, x.j);
+    // a suffix record matches only with a tail of exactly klen - l - 1 slices; the key's
+    // tail words for the first slab's data words 1 .. 30 load beside the record
+    const u32 want = rkind == kSuffix ? klen - l - 1 : 0u;
+    u32 kw[W];
+#pragma unroll
+    for (u32 e = 0; e < W; e++) {
+      const u32 d = w0 + e;
+      kw[e] = (d >= 1 && d <= want && d < kRecordWordsPerSlab) ? key[l + d] : 0u;
+    }
+    const u32 lw = x.template lane<0>(slot_lo(cur.s[0]));
     const u32 tl = rec_tail_len(lw), vl = rec_value_len(lw);
     const u32 total = 1 + tl + vl;
     const u32 vcopy = vl < maxV ? vl : maxV;
-    const u32* key = a.keys + i * (u64)maxK;
-    u32* out = a.out_values + i * (u64)maxV;
-    bool hit = !(rkind == kSuffix && tl != klen - l - 1);
-    for (u32 d0 = 0; hit;) {
-      // tail words precede value words: every tail word up to a slab's value words has
-      // been compared when they are copied
+    bool hit = !(rkind == kSuffix && tl != want);
+    // first slab: tail words (compared) precede value words (copied)
+    if (hit) {
+      bool mis = false;
+#pragma unroll
+      for (u32 e = 0; e < W; e++) {
+        const u32 d = w0 + e;
+        const u32 v = (e & 1) ? slot_hi(cur.s[e / 2]) : slot_lo(cur.s[e / 2]);
+        if (d >= 1 && d <= tl && d < kRecordWordsPerSlab) mis |= v != kw[e];
+      }
+      if (x.bor(mis ? 1u : 0u)) hit = false;
+    }
+    if (hit) {
+#pragma unroll
+      for (u32 e = 0; e < W; e++) {
+        const u32 d = w0 + e;
+        const u32 v = (e & 1) ? slot_hi(cur.s[e / 2]) : slot_lo(cur.s[e / 2]);
+        if (d < kRecordWordsPerSlab && d >= 1 + tl && d < 1 + tl + vcopy) out[d - 1 - tl] = v;
+      }
+    }
+    // further slabs (records of more than 31 words): key words loaded as compared
+    for (u32 d0 = kRecordWordsPerSlab; hit && d0 < total; d0 += kRecordWordsPerSlab) {
+      const u32 nxt = x.template lane<T - 1>(slot_hi(cur.s[S - 1]));  // word 31
+      if (nxt >= nslabs) {
+        if (x.j == 0) atomicCAS(tv.err, 0u, (u32)kErrBadIndex);
+        hit = false;
+        break;
+      }
+      cur = load_piece<T>(pool, nxt, x.j);
       bool mis = false;
       if (rkind == kSuffix) {
 #pragma unroll
-        for (u32 t = 0; t < S; t++) {
-#pragma unroll
-          for (u32 hh = 0; hh < 2; hh++) {
-            const u32 wd = 2 * (S * c.j + t) + hh;
-            const u32 d = d0 + wd;
-            const u32 v = hh ? slot_hi(cur.s[t]) : slot_lo(cur.s[t]);
-            if (wd < kRecordWordsPerSlab && d >= 1 && d <= tl) mis |= v != key[l + d];
-          }
+        for (u32 e = 0; e < W; e++) {
+          const u32 wd = w0 + e, d = d0 + wd;
+          const u32 v = (e & 1) ? slot_hi(cur.s[e / 2]) : slot_lo(cur.s[e / 2]);
+          if (wd < kRecordWordsPerSlab && d <= tl) mis |= v != key[l + d];
         }
       }
-      if (c.bits(mis)) {
+      if (x.bor(mis ? 1u : 0u)) {
         hit = false;
         break;
       }
 #pragma unroll
-      for (u32 t = 0; t < S; t++) {
-#pragma unroll
-        for (u32 hh = 0; hh < 2; hh++) {
-          const u32 wd = 2 * (S * c.j + t) + hh;
-          const u32 d = d0 + wd;
-          const u32 v = hh ? slot_hi(cur.s[t]) : slot_lo(cur.s[t]);
-          if (wd < kRecordWordsPerSlab && d >= 1 + tl && d < 1 + tl + vcopy) out[d - 1 - tl] = v;
-        }
+      for (u32 e = 0; e < W; e++) {
+        const u32 wd = w0 + e, d = d0 + wd;
+        const u32 v = (e & 1) ? slot_hi(cur.s[e / 2]) : slot_lo(cur.s[e / 2]);
+        if (wd < kRecordWordsPerSlab && d >= 1 + tl && d < 1 + tl + vcopy) out[d - 1 - tl] = v;
       }
-      d0 += kRecordWordsPerSlab;
-      if (d0 >= total) break;
-      const u32 next = c.from(T - 1, slot_hi(cur.s[S - 1]));  // word 31
-      if (next >= nslabs) {
-        if (c.j == 0) atomicCAS(tv.err, 0u, (u32)kErrBadIndex);
-        hit = false;
-        break;
-      }
-      cur = load_piece<T>(pool, next, c.j);
     }
-    if (c.j == 0) a.out_value_lengths[i] = hit ? vl : 0u;
+    if (x.j == 0) a.out_value_lengths[i] = hit ? vl : 0u;
   }
 }
 
