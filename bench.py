@@ -486,8 +486,8 @@ def main():
                          "known traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--find-config", default="",
-                    help="passes,find_tile,depth: pin the read-only find configuration "
-                         "instead of tuning it (for profiling runs)")
+                    help="passes,find_tile,depth[,record_tile]: pin the read-only find "
+                         "configuration instead of tuning it (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: every box in this pool has one GPU, so "
@@ -538,8 +538,11 @@ def main():
     # leaves the fastest setting applied
     if args.find_config:
         # a pinned configuration (profiling runs): the tuner measures only that one
-        fp, ft, fd = (int(v) for v in args.find_config.split(","))
-        tuner = Autotuner(tree, find_tiles=(ft,), find_depths=(fd,), find_passes=(fp,), trials=1)
+        pinned = [int(v) for v in args.find_config.split(",")]
+        fp, ft, fd = pinned[:3]
+        rt = pinned[3] if len(pinned) > 3 else 0
+        tuner = Autotuner(tree, find_tiles=(ft,), find_depths=(fd,), find_passes=(fp,),
+                          record_tiles=(rt,), trials=1)
     else:
         tuner = Autotuner(tree, find_tiles=(1, 2, 4, 8, 16), find_depths=(1, 2, 4),
                           find_passes=(1, 2), trials=1)
@@ -589,8 +592,9 @@ def main():
     traffic, traffic_note = None, None
     tpath = os.path.join(ROOT, "profiles", "find_kernel_16M_varlen_traffic.json")
     if best.get("find_passes", 1) == 2:
+        rtile = best.get("record_tile", 0) or best["find_tile"]
         kname = (f"narrow_descend_kernel<{best['find_tile']}, {best['find_depth']}> + "
-                 f"narrow_record_kernel<{best['find_tile']}>")
+                 f"narrow_record_kernel<{rtile}>")
     else:
         kname = f"narrow_find_kernel<{best['find_tile']}, {best['find_depth']}>"
     if os.path.exists(tpath):
@@ -620,6 +624,7 @@ def main():
             "max_value_length": mv, "tile_size": args.tile,
             "find_tile": best["find_tile"], "find_depth": best["find_depth"],
             "find_passes": best.get("find_passes", 1),
+            "record_tile": best.get("record_tile", 0) or best["find_tile"],
         },
         "insert_mops": round(n / t_ins / 1e3, 2),
         "erase_mops": round(n_erase / t_er / 1e3, 2),
@@ -636,7 +641,7 @@ def main():
             "traffic_source": traffic_note,
         },
         "find_tuning": [{"find_passes": c.get("find_passes", 1), "find_tile": c["find_tile"],
-                         "find_depth": c["find_depth"],
+                         "find_depth": c["find_depth"], "record_tile": c.get("record_tile"),
                          "mops": round(n / ms / 1e3, 1)} for _, _, c, ms in tuner.report()],
     }
     tree.close()

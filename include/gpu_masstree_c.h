@@ -155,6 +155,11 @@ int gmt_set_find_tile(gmt_index* index, uint32_t find_tile);
  * the reference; results do not depend on it. */
 int gmt_set_find_passes(gmt_index* index, uint32_t passes);
 
+/* Lanes per lookup of the record pass of two-pass finds (1, 2, 4, 8 or 16), or 0 (the
+ * default) to use the find tile.  A tuning knob with no counterpart in the reference; results
+ * do not depend on it. */
+int gmt_set_record_tile(gmt_index* index, uint32_t record_tile);
+
 /* Tile width (lanes per operation: 16, 32 or 64) that later calls on this index use.
  * The reference fixes it per instance as the tile_size template argument
  * (include/gpu_masstree.hpp:42-46); here all widths share one node layout and one tree

@@ -194,6 +194,12 @@ struct gpu_masstree {
   // Narrow finds in one kernel (1) or as a descent pass and a record pass (2).
   void set_find_passes(uint32_t p) { find_passes_ = p == 1 ? 1u : 2u; }
   uint32_t find_passes() const { return find_passes_; }
+  // Lanes per lookup of the record pass of two-pass finds: 1, 2, 4, 8 or 16, or 0 for
+  // the find tile.
+  void set_record_tile(uint32_t t) {
+    record_tile_ = (t == 1 || t == 2 || t == 4 || t == 8 || t == 16) ? t : 0;
+  }
+  uint32_t record_tile() const { return record_tile_; }
 
   template <bool update_if_exists = false, bool enable_suffix = true, bool use_shmem_key = false>
   void insert(const key_slice_type* keys, const size_type max_key_length,
@@ -625,11 +631,13 @@ struct gpu_masstree {
         case 4: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 4>, tiles * 4); break;
         default: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 2>, tiles * 2); break;
       }
-      auto rec = indexingpu_amd::narrow_record_kernel<T>;
-      const uint64_t want = (a.n + tiles - 1) / tiles;
-      const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(rec));
-      hipLaunchKernelGGL(rec, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream, view(),
-                         a, static_cast<const uint32_t*>(park_));
+      switch (record_tile_ ? record_tile_ : T) {
+        case 1: launch_record<1>(a, stream); break;
+        case 2: launch_record<2>(a, stream); break;
+        case 4: launch_record<4>(a, stream); break;
+        case 8: launch_record<8>(a, stream); break;
+        default: launch_record<16>(a, stream); break;
+      }
       return;
     }
     switch (find_depth_) {
@@ -638,6 +646,15 @@ struct gpu_masstree {
       case 8: launch_reader(indexingpu_amd::narrow_find_kernel<T, 8>, tiles * 8, a, stream); break;
       default: launch_reader(indexingpu_amd::narrow_find_kernel<T, 4>, tiles * 4, a, stream); break;
     }
+  }
+  template <uint32_t T>
+  void launch_record(const indexingpu_amd::batch_args& a, hipStream_t stream) {
+    constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / T;
+    auto rec = indexingpu_amd::narrow_record_kernel<T>;
+    const uint64_t want = (a.n + tiles - 1) / tiles;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(rec));
+    hipLaunchKernelGGL(rec, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream, view(),
+                       a, static_cast<const uint32_t*>(park_));
   }
   void launch_wide(const indexingpu_amd::batch_args& a, hipStream_t stream) {
     constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / tile_size;
@@ -685,6 +702,7 @@ struct gpu_masstree {
   uint32_t find_depth_ = 1;
   uint32_t find_tile_ = 2;
   uint32_t find_passes_ = 2;
+  uint32_t record_tile_ = 0;
   uint32_t* park_ = nullptr;
   uint64_t park_words_ = 0;
   uint32_t multi_grid_percent_ = 100;

@@ -7,6 +7,7 @@ The knobs (all switchable between calls on one index, same tree state):
     16 / width slots of a node; 0 = the tile width);
   * find depth: lookups each tile keeps in flight in read-only find batches (1/2/4/8);
   * find passes: read-only finds as one kernel or as a descent and a record pass;
+  * record tile: lanes per lookup of the record pass (1/2/4/8/16; 0 = the find tile);
   * grid share: percent of the resident-block capacity the batched form launches;
   * batched-form mode: one persistent launch with grid barriers, or one launch per batch.
 
@@ -34,6 +35,7 @@ TILES = (16, 32, 64)
 FIND_TILES = (1, 2, 4, 8, 16)
 FIND_DEPTHS = (1, 2, 4, 8)
 FIND_PASSES = (1, 2)
+RECORD_TILES = (0, 2, 4, 8)
 GRID_PERCENTS = (100, 50, 25)
 
 
@@ -80,9 +82,11 @@ class Autotuner:
     """
 
     def __init__(self, index, tiles=TILES, find_tiles=FIND_TILES, find_depths=FIND_DEPTHS,
-                 grid_percents=GRID_PERCENTS, trials=3, timer=None, find_passes=FIND_PASSES):
+                 grid_percents=GRID_PERCENTS, trials=3, timer=None, find_passes=FIND_PASSES,
+                 record_tiles=RECORD_TILES):
         self.index = index
         self.find_passes = tuple(find_passes)
+        self.record_tiles = tuple(record_tiles)
         self.tiles = tuple(tiles)
         self.find_tiles = tuple(find_tiles)
         self.find_depths = tuple(find_depths)
@@ -100,6 +104,8 @@ class Autotuner:
             self.index.set_find_tile(cfg["find_tile"])
         if "find_passes" in cfg:
             self.index.set_find_passes(cfg["find_passes"])
+        if "record_tile" in cfg:
+            self.index.set_record_tile(cfg["record_tile"])
         if "find_depth" in cfg:
             self.index.set_find_depth(cfg["find_depth"])
         if "grid_percent" in cfg:
@@ -129,16 +135,28 @@ class Autotuner:
     def tune_find(self, run_find, max_key_length, warmup=1):
         """run_find() launches one read-only find batch (the same batch every call).
         Every (passes, find tile, depth) candidate is warmed up, then timed `trials` times; the
-        minimum counts."""
+        minimum counts.  When two passes win, a second stage tries each record tile with
+        the winning find tile and depth (the first stage runs with the record tile of the
+        first entry of record_tiles)."""
+        def measure(cfg):
+            self.apply_config(cfg)
+            for _ in range(warmup):
+                self.timer(run_find)
+            return min(self.timer(run_find) for _ in range(self.trials))
+
+        r0 = self.record_tiles[0] if self.record_tiles else 0
         scored = []
         for passes, tile, depth in itertools.product(self.find_passes, self.find_tiles,
                                                      self.find_depths):
             cfg = {"find_passes": passes, "find_tile": tile, "find_depth": depth}
-            self.apply_config(cfg)
-            for _ in range(warmup):
-                self.timer(run_find)
-            ms = min(self.timer(run_find) for _ in range(self.trials))
-            scored.append((cfg, ms))
+            if passes == 2:
+                cfg["record_tile"] = r0
+            scored.append((cfg, measure(cfg)))
+        best = min(scored, key=lambda cs: cs[1])[0]
+        if best["find_passes"] == 2:
+            for r in self.record_tiles[1:]:
+                cfg = dict(best, record_tile=r)
+                scored.append((cfg, measure(cfg)))
         return self._pick(("find", length_class(max_key_length)), scored)
 
     def tune_online(self, op, run_unit, max_key_length, space=None):

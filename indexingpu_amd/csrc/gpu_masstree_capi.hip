@@ -318,6 +318,15 @@ int gmt_set_find_passes(gmt_index* index, uint32_t passes) {
   return GMT_OK;
 }
 
+int gmt_set_record_tile(gmt_index* index, uint32_t record_tile) {
+  if (!index) return bad("gmt_set_record_tile: null handle");
+  if (record_tile != 0 && record_tile != 1 && record_tile != 2 && record_tile != 4 &&
+      record_tile != 8 && record_tile != 16)
+    return bad("gmt_set_record_tile: record tile must be 0, 1, 2, 4, 8 or 16");
+  B(index)->on_all([&](auto& t) { t.set_record_tile(record_tile); });
+  return GMT_OK;
+}
+
 int gmt_set_multi_mode(gmt_index* index, uint32_t mode) {
   if (!index) return bad("gmt_set_multi_mode: null handle");
   if (mode > 1) return bad("gmt_set_multi_mode: mode must be 0 or 1");

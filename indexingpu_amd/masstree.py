@@ -74,6 +74,7 @@ def load_library():
     lib.gmt_set_tile_size.argtypes = [vp, u32]
     lib.gmt_set_find_tile.argtypes = [vp, u32]
     lib.gmt_set_find_passes.argtypes = [vp, u32]
+    lib.gmt_set_record_tile.argtypes = [vp, u32]
     lib.gmt_tile_size.argtypes = [vp]
     lib.gmt_tile_size.restype = u32
     lib.gmt_last_error_message.restype = ctypes.c_char_p
@@ -271,12 +272,18 @@ class GpuMasstree:
         self._check(self._lib.gmt_set_tile_size(self._h, int(t)))
 
     def set_find_tile(self, t):
-        """Lanes per lookup of read-only find batches: 4, 8, 16, or 0 (the tile width)."""
+        """Lanes per lookup of read-only find batches: 1, 2, 4, 8, 16, or 0 (the tile
+        width)."""
         self._check(self._lib.gmt_set_find_tile(self._h, int(t)))
 
     def set_find_passes(self, p):
         """Read-only finds as one kernel (1) or descent + record passes (2)."""
         self._check(self._lib.gmt_set_find_passes(self._h, int(p)))
+
+    def set_record_tile(self, t):
+        """Lanes per lookup of the record pass of two-pass finds: 1, 2, 4, 8, 16, or 0
+        (the find tile)."""
+        self._check(self._lib.gmt_set_record_tile(self._h, int(t)))
 
     def set_find_depth(self, k):
         """Lookups each tile keeps in flight in read-only find batches (1, 2, 4, 8)."""

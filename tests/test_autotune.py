@@ -10,6 +10,7 @@ class FakeIndex:
         self.grid = 100
         self.ftile = 8
         self.passes = 2
+        self.rtile = 0
         self.log = []
 
     def set_find_tile(self, t):
@@ -17,6 +18,9 @@ class FakeIndex:
 
     def set_find_passes(self, p):
         self.passes = p
+
+    def set_record_tile(self, t):
+        self.rtile = t
 
     def set_find_depth(self, k):
         self.depth = k
@@ -36,7 +40,8 @@ class FakeClock:
         fn()
         self.calls += 1
         i = self.index
-        return self.cost(i.tile_size, i.depth, i.grid, i.ftile) + (0.5 if i.passes == 1 else 0.0)
+        extra = 0.5 if i.passes == 1 else (0.0 if i.rtile == 4 else 0.25)
+        return self.cost(i.tile_size, i.depth, i.grid, i.ftile) + extra
 
 
 def test_length_class():
@@ -51,13 +56,23 @@ def test_tune_find_picks_minimum_and_applies_it():
     tu = Autotuner(idx, trials=2, timer=clock)
     runs = []
     cfg = tu.tune_find(lambda: runs.append((idx.ftile, idx.depth)), max_key_length=8)
-    assert cfg == {"find_passes": 2, "find_tile": 8, "find_depth": 2}
-    assert (idx.passes, idx.ftile, idx.depth) == (2, 8, 2)
-    # 2 pass counts x 5 tiles x 4 depths, each 1 warmup + 2 timed runs
-    assert len(runs) == 40 * 3 and clock.calls == 40 * 3
+    assert cfg == {"find_passes": 2, "find_tile": 8, "find_depth": 2, "record_tile": 4}
+    assert (idx.passes, idx.ftile, idx.depth, idx.rtile) == (2, 8, 2, 4)
+    # 2 pass counts x 5 tiles x 4 depths, then the 3 other record tiles with the best
+    # two-pass setting; each 1 warmup + 2 timed runs
+    assert len(runs) == 43 * 3 and clock.calls == 43 * 3
     assert tu.best("find", 7) == cfg          # same length class (8)
     assert tu.best("find", 9) is None          # class 16 not tuned
-    assert len(tu.report()) == 40
+    assert len(tu.report()) == 43
+
+
+def test_tune_find_skips_record_stage_when_one_pass_wins():
+    idx = FakeIndex()
+    clock = FakeClock(idx, lambda t, d, g, f: abs(f - 4) + abs(d - 1) + 1.0)
+    tu = Autotuner(idx, trials=1, timer=clock, find_passes=(1,))
+    cfg = tu.tune_find(lambda: None, max_key_length=4)
+    assert cfg == {"find_passes": 1, "find_tile": 4, "find_depth": 1}
+    assert len(tu.report()) == 20
 
 
 def test_tune_online_scores_per_operation_and_never_reuses_units():

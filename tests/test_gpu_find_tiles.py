@@ -58,3 +58,19 @@ def test_find_tiles_match_oracle(loaded, find_tile, depth, max_value, passes):
     exp_v, exp_l = o.find(keys, lens, mk, max_value)
     assert np.array_equal(got_l, exp_l)
     assert np.array_equal(got_v, exp_v)
+
+
+@pytest.mark.parametrize("record_tile", [0, 1, 2, 4, 8, 16])
+@pytest.mark.parametrize("find_tile", [2, 4])
+@pytest.mark.parametrize("max_value", [70, 3])
+def test_record_tiles_match_oracle(loaded, record_tile, find_tile, max_value):
+    """Two-pass finds with the record pass at every tile width (0 = the find tile)."""
+    g, o, keys, lens, mk = loaded
+    g.t.set_find_passes(2)
+    g.t.set_find_tile(find_tile)
+    g.t.set_find_depth(1)
+    g.t.set_record_tile(record_tile)
+    got_v, got_l = g.find(keys, lens, mk, max_value)
+    exp_v, exp_l = o.find(keys, lens, mk, max_value)
+    assert np.array_equal(got_l, exp_l)
+    assert np.array_equal(got_v, exp_v)
