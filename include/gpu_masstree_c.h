@@ -160,6 +160,14 @@ int gmt_set_find_passes(gmt_index* index, uint32_t passes);
  * do not depend on it. */
 int gmt_set_record_tile(gmt_index* index, uint32_t record_tile);
 
+/* Mixed batches of at least min_requests requests (default 16384) run their finds as a
+ * read-only pass with the narrow two-pass kernels, then the other requests; 0 splits
+ * every batch, 0xFFFFFFFF none.  The first split batch at a new largest size allocates
+ * 4 bytes per request of scratch and synchronizes.  A tuning knob with no counterpart in
+ * the reference; the outcome equals that of one launch, since a mixed batch writes no
+ * key twice and reads no key it writes. */
+int gmt_set_mixed_split(gmt_index* index, uint32_t min_requests);
+
 /* Tile width (lanes per operation: 16, 32 or 64) that later calls on this index use.
  * The reference fixes it per instance as the tile_size template argument
  * (include/gpu_masstree.hpp:42-46); here all widths share one node layout and one tree

@@ -296,8 +296,27 @@ struct gpu_masstree {
     a.out_values = values;
     a.out_value_lengths = value_lengths;
     a.results = results;
+    if (num_requests >= mixed_split_min_) {
+      // the finds as a read-only pass (narrow two-pass kernels, which leave every other
+      // request alone), then the other requests; stream order separates the passes.  A
+      // batch writes no key twice and reads no key it writes, so the outcome equals
+      // that of one launch running all requests together.
+      switch (find_tile_) {
+        case 1: launch_narrow<1>(a, stream, true); break;
+        case 4: launch_narrow<4>(a, stream, true); break;
+        case 8: launch_narrow<8>(a, stream, true); break;
+        case 16: launch_narrow<16>(a, stream, true); break;
+        default: launch_narrow<2>(a, stream, true); break;
+      }
+      a.skip_finds = 1;
+    }
     launch_batch<true>(a, stream, true);
   }
+
+  // Mixed batches of at least this many requests run their finds as a separate
+  // read-only pass (0: every batch; UINT32_MAX: none).
+  void set_mixed_split(uint32_t min_requests) { mixed_split_min_ = min_requests; }
+  uint32_t mixed_split() const { return mixed_split_min_; }
 
   // Batched form (no counterpart in the reference): applies `num_batches` mixed batches
   // in order within one persistent launch.  The arrays hold the requests of every batch
@@ -615,9 +634,10 @@ struct gpu_masstree {
                        view(), a);
   }
   template <uint32_t T>
-  void launch_narrow(const indexingpu_amd::batch_args& a, hipStream_t stream) {
+  void launch_narrow(const indexingpu_amd::batch_args& a, hipStream_t stream,
+                     bool two_pass = false) {
     constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / T;
-    if (find_passes_ == 2 && a.n > 0) {
+    if ((two_pass || find_passes_ == 2) && a.n > 0) {
       ensure_park(a.n);
       auto launch_descend = [&](auto kernel, uint32_t per_block) {
         const uint64_t want = (a.n + per_block - 1) / per_block;
@@ -703,6 +723,7 @@ struct gpu_masstree {
   uint32_t find_tile_ = 2;
   uint32_t find_passes_ = 2;
   uint32_t record_tile_ = 0;
+  uint32_t mixed_split_min_ = 1u << 14;
   uint32_t* park_ = nullptr;
   uint64_t park_words_ = 0;
   uint32_t multi_grid_percent_ = 100;

@@ -59,6 +59,9 @@ struct batch_args {
   // when set, the launch covers requests [range[0], range[1]) clamped to n, and the
   // grid need not match the batch size.
   const u64* range;
+  // Mixed batches run as a read-only pass over their finds followed by a pass over the
+  // rest: with skip_finds set (and types given), batch_kernel leaves find requests alone.
+  u32 skip_finds;
 };
 
 __device__ __forceinline__ u64 request_index(const batch_args& a, u64 i) {
@@ -101,6 +104,21 @@ batch_kernel(tree_view tv, batch_args a) {
   u32 ri = 0;  // request_index(a, i); batch launches hold fewer than 2^32 requests
   op_state<gpu_node> st;
   request rq;
+  // with skip_finds, the tile's first 16 lanes look at the next 16 requests the tile
+  // would take and it moves to the first that is not a find
+  auto seek = [&]() {
+    if (!a.skip_finds || !a.types) return;
+    while (i < hi) {
+      const u64 e = i + (u64)c.slot * ntiles;
+      const u32 m = c.ballot(e < hi && a.types[request_index(a, e)] != op_find);
+      if (m) {
+        i += (u64)(__ffs(m) - 1) * ntiles;
+        return;
+      }
+      i += 16 * ntiles;
+    }
+  };
+  seek();
   bool active = i < hi;
   auto start = [&]() {
     ri = (u32)request_index(a, i);
@@ -152,6 +170,7 @@ batch_kernel(tree_view tv, batch_args a) {
         }
       }
       i += ntiles;
+      seek();
       active = i < hi;
       if (active) start();
     }

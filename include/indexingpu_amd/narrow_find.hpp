@@ -305,6 +305,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_find_kernel(tree_view tv
 // Results equal the one-pass kernel's; values of absent keys are not written by either
 // pass.
 constexpr u32 kPendingRecord = 0x80000000u;  // record indices stay below 2^31
+constexpr u32 kNoOutput = 0xFFFFFFFEu;       // a request the find passes do not answer
 
 // Exchanges inside a tile for the descent.  Tiles of two or four lanes use DPP quad
 // permutations (an operand modifier of a VALU instruction: no LDS traffic, no wait);
@@ -391,6 +392,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   u64 idx[K];
   u32 node[K], layer[K], klen[K], q[K];
   u32 next_len[K], next_q[K];  // length and first slice of lookup idx[k], loaded ahead
+  u32 next_kind[K], kind[K];   // request kinds (mixed batches: only finds are looked up)
   bool active[K], q_stale[K];
 
   // loads of the tile's following lookup (the index clamped into the batch, so the loads
@@ -399,6 +401,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
     const u64 c = i < a.n ? i : a.n - 1;
     next_len[k] = a.key_lengths ? a.key_lengths[c] : maxK;
     next_q[k] = a.keys[c * (u64)maxK];
+    if (a.types) next_kind[k] = a.types[c];
   };
   // begins lookup idx[k] (its length and first slice are in next_len/next_q, loaded one
   // lookup ahead) and loads those of the tile's following lookup, which arrive while
@@ -412,6 +415,11 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
     asm volatile("v_mov_b32 %0, %1" : "=v"(q0) : "v"(next_q[k]));
     klen[k] = L;
     q[k] = q0;
+    if (a.types) {
+      u32 kd;
+      asm volatile("v_mov_b32 %0, %1" : "=v"(kd) : "v"(next_kind[k]));
+      kind[k] = kd;
+    }
     node[k] = tv.root;
     layer[k] = 0;
     q_stale[k] = false;
@@ -477,7 +485,9 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       const u32 v = x.bor(mine == gid ? cand : 0u);
       const bool found = gid < nk;
       u32 len = kInvalid;  // kInvalid: the lookup goes on
-      if (klen[k] == 0 || klen[k] > maxK) {
+      if (a.types && kind[k] != op_find) {
+        len = kNoOutput;  // not a find: left to the pass that runs the other requests
+      } else if (klen[k] == 0 || klen[k] > maxK) {
         if (x.j == 0) atomicCAS(tv.err, 0u, (u32)kErrBadLength);
         len = 0;
       } else if (!border) {
@@ -508,7 +518,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
         len = 0;
       }
       if (len != kInvalid) {
-        if (x.j == 0) a.out_value_lengths[idx[k]] = len;
+        if (x.j == 0 && len != kNoOutput) a.out_value_lengths[idx[k]] = len;
         idx[k] += stride;
         start(k);
       }
@@ -534,22 +544,25 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_record_kernel(tree_view 
   const u32 w0 = W * x.j;  // this lane's first word of a slab
   if (a.n == 0) return;
   u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
-  u32 next_w, next_pk, next_len;
+  u32 next_w, next_pk, next_len, next_kind = op_find;
   auto fetch = [&](u64 e) {
     const u64 c = e < a.n ? e : a.n - 1;  // clamped: the loads are unconditional
     next_w = a.out_value_lengths[c];
     next_pk = park[c];
     next_len = a.key_lengths ? a.key_lengths[c] : maxK;
+    if (a.types) next_kind = a.types[c];
   };
   fetch(i);
   for (; i < a.n; i += ntiles) {
     // copies made before the next entry's loads are issued into the same registers
-    u32 w, pk, klen;
+    u32 w, pk, klen, kd = op_find;
     asm volatile("v_mov_b32 %0, %1" : "=v"(w) : "v"(next_w));
     asm volatile("v_mov_b32 %0, %1" : "=v"(pk) : "v"(next_pk));
     asm volatile("v_mov_b32 %0, %1" : "=v"(klen) : "v"(next_len));
+    if (a.types) asm volatile("v_mov_b32 %0, %1" : "=v"(kd) : "v"(next_kind));
     fetch(i + ntiles);
-    if (!(w & kPendingRecord)) continue;
+    // (in a mixed batch the length word of a write request is its input value length)
+    if (!(w & kPendingRecord) || kd != op_find) continue;
     const u32 rec = w & ~kPendingRecord;  // below the pool size: checked by pass 1
     const u32 rkind = pk >> 16, l = pk & 0xFFFFu;
     const u32* key = a.keys + i * (u64)maxK;

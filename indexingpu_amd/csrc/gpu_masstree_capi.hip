@@ -327,6 +327,12 @@ int gmt_set_record_tile(gmt_index* index, uint32_t record_tile) {
   return GMT_OK;
 }
 
+int gmt_set_mixed_split(gmt_index* index, uint32_t min_requests) {
+  if (!index) return bad("gmt_set_mixed_split: null handle");
+  B(index)->on_all([&](auto& t) { t.set_mixed_split(min_requests); });
+  return GMT_OK;
+}
+
 int gmt_set_multi_mode(gmt_index* index, uint32_t mode) {
   if (!index) return bad("gmt_set_multi_mode: null handle");
   if (mode > 1) return bad("gmt_set_multi_mode: mode must be 0 or 1");

@@ -75,6 +75,7 @@ def load_library():
     lib.gmt_set_find_tile.argtypes = [vp, u32]
     lib.gmt_set_find_passes.argtypes = [vp, u32]
     lib.gmt_set_record_tile.argtypes = [vp, u32]
+    lib.gmt_set_mixed_split.argtypes = [vp, u32]
     lib.gmt_tile_size.argtypes = [vp]
     lib.gmt_tile_size.restype = u32
     lib.gmt_last_error_message.restype = ctypes.c_char_p
@@ -284,6 +285,11 @@ class GpuMasstree:
         """Lanes per lookup of the record pass of two-pass finds: 1, 2, 4, 8, 16, or 0
         (the find tile)."""
         self._check(self._lib.gmt_set_record_tile(self._h, int(t)))
+
+    def set_mixed_split(self, min_requests):
+        """Mixed batches of at least min_requests requests run their finds as a separate
+        read-only pass (0: every batch; 0xFFFFFFFF: none)."""
+        self._check(self._lib.gmt_set_mixed_split(self._h, int(min_requests) & 0xFFFFFFFF))
 
     def set_find_depth(self, k):
         """Lookups each tile keeps in flight in read-only find batches (1, 2, 4, 8)."""

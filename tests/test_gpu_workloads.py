@@ -54,13 +54,16 @@ def test_cpu_path_shape_fixed_length():
     assert int((el > 0).sum()) == n // 2
 
 
+@pytest.mark.parametrize("split", [0, 0xFFFFFFFF], ids=["finds_pass", "one_launch"])
 @pytest.mark.parametrize("tile", [16, 32])
-def test_mixed_batches_and_scans(tile):
+def test_mixed_batches_and_scans(tile, split):
     """Mixed shape: Zipf(0.99) finds / fresh inserts / erases of unread keys, then range
-    scans with random lower bounds and per-query limits in [1, 100]."""
+    scans with random lower bounds and per-query limits in [1, 100]; mixed batches run as
+    one launch or as a read-only pass over their finds followed by the other requests."""
     wl = W.MixedWorkload(200_000, key_len=4, seed=13)
     mk, mv = 4, 2
     g, o = S.GpuBackend(tile, pool_bytes=256 << 20), S.OracleBackend()
+    g.t.set_mixed_split(split)
     _eq(g.insert(wl.keys, wl.lens, wl.values, None, mk, mv),
         o.insert(wl.keys, wl.lens, wl.values, None, mk, mv))
     for _ in range(4):
@@ -176,3 +179,46 @@ def test_key_ordered_insert_batches(order):
     assert g.items() == o.items()
     ok, *_ = g.t.validate()
     assert ok
+
+
+@pytest.mark.parametrize("find_tile", [1, 2, 4])
+def test_mixed_finds_pass_long_keys_and_values(find_tile):
+    """Mixed batches whose finds run as a read-only pass, over keys of 1..40 slices that
+    share prefixes (trie layers, suffix records) and values of 0..70 slices (records of
+    several slabs): finds of present and absent keys, inserts of fresh keys with long
+    values, erases of present keys none of the finds read."""
+    rng = np.random.default_rng(31)
+    mk, mv = 40, 70
+    base, blen = W.unique_random_keys(rng, 6000, 1, mk)
+    keys, lens = base.copy(), blen.copy()
+    donor = rng.integers(0, len(lens), size=len(lens))
+    share = rng.integers(0, mk, size=len(lens))
+    for i in range(0, len(lens), 3):
+        p = min(share[i], lens[i] - 1, lens[donor[i]])
+        keys[i, :p] = base[donor[i], :p]
+    _, first = np.unique(W.key_hash64(keys, lens), return_index=True)
+    first = np.sort(first)
+    keys, lens = keys[first], lens[first]
+    n = len(lens)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    vl = rng.choice([0, 1, 2, 5, 30, 31, 62, 70], size=n).astype(np.uint32)
+    g, o = S.GpuBackend(16, pool_bytes=256 << 20), S.OracleBackend()
+    g.t.set_mixed_split(0)
+    g.t.set_find_tile(find_tile)
+    pre = n // 2
+    _eq(g.insert(keys[:pre], lens[:pre], vals[:pre], vl[:pre], mk, mv),
+        o.insert(keys[:pre], lens[:pre], vals[:pre], vl[:pre], mk, mv))
+    perm_pre = rng.permutation(pre)
+    perm_new = pre + rng.permutation(n - pre)
+    finds = np.concatenate([perm_pre[: pre // 2], perm_new[: (n - pre) // 3]])
+    erases = perm_pre[pre // 2:]
+    inserts = perm_new[(n - pre) // 3:]
+    idx = np.concatenate([finds, erases, inserts])
+    types = np.concatenate([np.full(len(finds), 2), np.full(len(erases), 1),
+                            np.full(len(inserts), 0)]).astype(np.uint8)
+    order = rng.permutation(len(idx))
+    idx, types = idx[order], types[order]
+    bk, bl, bv, bvl = keys[idx], lens[idx], vals[idx], vl[idx]
+    for a, b in zip(g.mixed(types, bk, bl, bv, bvl, mk, mv), o.mixed(types, bk, bl, bv, bvl, mk, mv)):
+        _eq(a, b)
+    assert g.items() == o.items()
