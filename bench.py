@@ -350,6 +350,7 @@ def run_batched(torch, args):
         if way == "plain":
             ms = timed_ms(lambda: per_batch_calls(tree, v, vl, r))
         elif way == "graph":
+            tree.reserve_find_scratch(max(b - a for a, b in zip(o[:-1], o[1:])))
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
                 per_batch_calls(tree, v, vl, r)

@@ -168,6 +168,12 @@ int gmt_set_record_tile(gmt_index* index, uint32_t record_tile);
  * key twice and reads no key it writes. */
 int gmt_set_mixed_split(gmt_index* index, uint32_t min_requests);
 
+/* Allocates now the scratch that two-pass finds and split mixed batches use (4 bytes per
+ * request) for batches of up to max_requests requests; without it the first larger
+ * batch allocates and synchronizes.  Call it before capturing such calls into a graph.
+ * No counterpart in the reference. */
+int gmt_reserve_find_scratch(gmt_index* index, uint64_t max_requests);
+
 /* Tile width (lanes per operation: 16, 32 or 64) that later calls on this index use.
  * The reference fixes it per instance as the tile_size template argument
  * (include/gpu_masstree.hpp:42-46); here all widths share one node layout and one tree

@@ -301,17 +301,16 @@ struct gpu_masstree {
       // request alone), then the other requests; stream order separates the passes.  A
       // batch writes no key twice and reads no key it writes, so the outcome equals
       // that of one launch running all requests together.
-      switch (find_tile_) {
-        case 1: launch_narrow<1>(a, stream, true); break;
-        case 4: launch_narrow<4>(a, stream, true); break;
-        case 8: launch_narrow<8>(a, stream, true); break;
-        case 16: launch_narrow<16>(a, stream, true); break;
-        default: launch_narrow<2>(a, stream, true); break;
-      }
+      launch_finds_pass(a, stream);
       a.skip_finds = 1;
     }
     launch_batch<true>(a, stream, true);
   }
+
+  // Allocates the two-pass find scratch for batches of up to max_requests requests now
+  // (it otherwise grows at the first larger batch, which synchronizes; callers that
+  // capture launches into a graph reserve first).
+  void reserve_find_scratch(uint64_t max_requests) { ensure_park(max_requests); }
 
   // Mixed batches of at least this many requests run their finds as a separate
   // read-only pass (0: every batch; UINT32_MAX: none).
@@ -341,19 +340,61 @@ struct gpu_masstree {
     m.a.results = results;
     if (multi_mode_ == 1 && total_requests < (1ull << 32)) {
       // one batch-kernel launch per batch, each reading its range from `offsets` on the
-      // device; stream order separates the batches, and the records they retire are
-      // drained once at the end (no launch of this call reads a record after it)
+      // device; stream order separates the batches, and the records each batch retires
+      // are drained right after it
       m.a.results = results;
       auto kernel = indexingpu_amd::batch_kernel<tile_size, true>;
       const uint32_t blocks = std::max<uint32_t>(
           1, (uint32_t)((uint64_t)indexingpu_amd::resident_blocks(kernel) * multi_grid_percent_ / 100));
+      // Batches of mixed_split_min_ requests or more run their finds as a read-only
+      // pass first, as mixed_batch does.  Outside graph capture the offsets are read
+      // back once, so each batch's grids fit its size and the split is decided per
+      // batch; under capture the launches take full grids and the split follows the
+      // average batch size.
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      IGPU_HIP_TRY(hipStreamIsCapturing(stream, &cs));
+      std::vector<uint64_t> off;
+      if (cs == hipStreamCaptureStatusNone) {
+        off.resize((size_t)num_batches + 1);
+        IGPU_HIP_TRY(hipMemcpyAsync(off.data(), offsets, off.size() * 8, hipMemcpyDeviceToHost,
+                                    stream));
+        IGPU_HIP_TRY(hipStreamSynchronize(stream));
+      }
+      const bool split_all = mixed_split_min_ != 0xFFFFFFFFu &&
+                             total_requests >= (uint64_t)mixed_split_min_ * num_batches;
+      // the find scratch is indexed from each batch's first request
+      uint64_t largest = total_requests;
+      if (!off.empty()) {
+        largest = 0;
+        for (uint32_t b = 0; b < num_batches; b++)
+          largest = std::max<uint64_t>(largest, off[b + 1] - std::min(off[b], off[b + 1]));
+        largest = std::min<uint64_t>(largest, total_requests);
+      }
+      if (mixed_split_min_ != 0xFFFFFFFFu) ensure_park(largest);
+      const uint32_t tiles_per_block = indexingpu_amd::kBlockThreads / tile_size;
       for (uint32_t b = 0; b < num_batches; b++) {
         indexingpu_amd::batch_args a = m.a;
         a.range = offsets + b;
-        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
+        uint64_t nb = total_requests;  // requests of this batch (an upper bound under capture)
+        bool split = split_all;
+        if (!off.empty()) {
+          const uint64_t hi = std::min<uint64_t>(off[b + 1], total_requests);
+          nb = hi - std::min<uint64_t>(off[b], hi);
+          if (nb == 0) continue;
+          split = mixed_split_min_ != 0xFFFFFFFFu && nb >= mixed_split_min_;
+        }
+        if (split) {
+          launch_finds_pass(a, stream, nb);
+          a.skip_finds = 1;
+        }
+        const uint32_t bb =
+            (uint32_t)std::min<uint64_t>(blocks, (nb + tiles_per_block - 1) / tiles_per_block);
+        hipLaunchKernelGGL(kernel, dim3(bb), dim3(indexingpu_amd::kBlockThreads), 0, stream,
                            view(), a);
+        // slabs of the records this batch retired go back to the allocator before the
+        // next batch's inserts take slabs (as with one mixed_batch call per batch)
+        drain(stream);
       }
-      drain(stream);
       return;
     }
     m.offsets = offsets;
@@ -633,14 +674,27 @@ struct gpu_masstree {
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
                        view(), a);
   }
+  // read-only pass over the finds of a mixed batch (or of the range a.range names, of
+  // at most `work` requests: the grids are sized to it)
+  void launch_finds_pass(const indexingpu_amd::batch_args& a, hipStream_t stream,
+                         uint64_t work = 0) {
+    switch (find_tile_) {
+      case 1: launch_narrow<1>(a, stream, true, work); break;
+      case 4: launch_narrow<4>(a, stream, true, work); break;
+      case 8: launch_narrow<8>(a, stream, true, work); break;
+      case 16: launch_narrow<16>(a, stream, true, work); break;
+      default: launch_narrow<2>(a, stream, true, work); break;
+    }
+  }
   template <uint32_t T>
   void launch_narrow(const indexingpu_amd::batch_args& a, hipStream_t stream,
-                     bool two_pass = false) {
+                     bool two_pass = false, uint64_t work = 0) {
     constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / T;
+    if (work == 0) work = a.n;
     if ((two_pass || find_passes_ == 2) && a.n > 0) {
-      ensure_park(a.n);
+      ensure_park(a.range ? work : a.n);  // indexed from the first request of the range
       auto launch_descend = [&](auto kernel, uint32_t per_block) {
-        const uint64_t want = (a.n + per_block - 1) / per_block;
+        const uint64_t want = (work + per_block - 1) / per_block;
         const uint32_t blocks =
             (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(kernel));
         hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
@@ -652,11 +706,11 @@ struct gpu_masstree {
         default: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 2>, tiles * 2); break;
       }
       switch (record_tile_ ? record_tile_ : T) {
-        case 1: launch_record<1>(a, stream); break;
-        case 2: launch_record<2>(a, stream); break;
-        case 4: launch_record<4>(a, stream); break;
-        case 8: launch_record<8>(a, stream); break;
-        default: launch_record<16>(a, stream); break;
+        case 1: launch_record<1>(a, stream, work); break;
+        case 2: launch_record<2>(a, stream, work); break;
+        case 4: launch_record<4>(a, stream, work); break;
+        case 8: launch_record<8>(a, stream, work); break;
+        default: launch_record<16>(a, stream, work); break;
       }
       return;
     }
@@ -668,10 +722,10 @@ struct gpu_masstree {
     }
   }
   template <uint32_t T>
-  void launch_record(const indexingpu_amd::batch_args& a, hipStream_t stream) {
+  void launch_record(const indexingpu_amd::batch_args& a, hipStream_t stream, uint64_t work) {
     constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / T;
     auto rec = indexingpu_amd::narrow_record_kernel<T>;
-    const uint64_t want = (a.n + tiles - 1) / tiles;
+    const uint64_t want = (work + tiles - 1) / tiles;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(rec));
     hipLaunchKernelGGL(rec, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream, view(),
                        a, static_cast<const uint32_t*>(park_));

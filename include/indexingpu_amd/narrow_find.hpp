@@ -300,10 +300,11 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_find_kernel(tree_view tv
 // so one combined kernel executes the interior, border and record code paths on nearly
 // every step.  Here pass 1 (narrow_descend_kernel) walks interior and border nodes
 // through the trie layers and parks each lookup that ends at a record:
-// out_value_lengths[i] = kPendingRecord | record index, and park[i] = keystate << 16 |
-// layer.  Pass 2 (narrow_record_kernel) reads every parked record in one uniform sweep.
-// Results equal the one-pass kernel's; values of absent keys are not written by either
-// pass.
+// out_value_lengths[i] = kPendingRecord | record index, and park[i - lo] = keystate << 16
+// | layer, lo being the first request of the launch (of its range, in the batched form).
+// Pass 2 (narrow_record_kernel) reads every parked record in one uniform sweep.  Results
+// equal the one-pass kernel's; values of absent keys are not written by either pass.
+// In a mixed batch both passes answer the finds only.
 constexpr u32 kPendingRecord = 0x80000000u;  // record indices stay below 2^31
 constexpr u32 kNoOutput = 0xFFFFFFFEu;       // a request the find passes do not answer
 
@@ -388,7 +389,13 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   const u64 first = ((u64)blockIdx.x * tiles_per_block + threadIdx.x / T) * K;
   const u32 maxK = a.max_key_length, maxV = a.max_value_length;
   const u32 slot0 = S * x.j;  // this lane's first slot
-  if (a.n == 0) return;
+  // requests [lo, hi): the batch, or the range read on the device (batched form)
+  u64 lo = 0, hi = a.n;
+  if (a.range) {
+    hi = min(a.range[1], a.n);
+    lo = min(a.range[0], hi);
+  }
+  if (lo >= hi) return;
   u64 idx[K];
   u32 node[K], layer[K], klen[K], q[K];
   u32 next_len[K], next_q[K];  // length and first slice of lookup idx[k], loaded ahead
@@ -398,7 +405,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   // loads of the tile's following lookup (the index clamped into the batch, so the loads
   // are unconditional and land in the registers the next start reads)
   auto fetch = [&](u32 k, u64 i) {
-    const u64 c = i < a.n ? i : a.n - 1;
+    const u64 c = i < hi ? i : hi - 1;
     next_len[k] = a.key_lengths ? a.key_lengths[c] : maxK;
     next_q[k] = a.keys[c * (u64)maxK];
     if (a.types) next_kind[k] = a.types[c];
@@ -423,13 +430,13 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
     node[k] = tv.root;
     layer[k] = 0;
     q_stale[k] = false;
-    active[k] = idx[k] < a.n;
+    active[k] = idx[k] < hi;
     fetch(k, idx[k] + stride);
   };
 
 #pragma unroll
   for (u32 k = 0; k < K; k++) {
-    idx[k] = first + k;
+    idx[k] = lo + first + k;
     fetch(k, idx[k]);
     start(k);
   }
@@ -507,7 +514,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
           if (x.j == 0 && maxV >= 1) a.out_values[idx[k] * (u64)maxV] = v;
           len = 1;
         } else if (v < nslabs) {
-          if (x.j == 0) park[idx[k]] = (ks << 16) | l;
+          if (x.j == 0) park[idx[k] - lo] = (ks << 16) | l;
           len = kPendingRecord | v;
         } else {
           node[k] = v;  // reported below
@@ -542,18 +549,23 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_record_kernel(tree_view 
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
   const u32 maxK = a.max_key_length, maxV = a.max_value_length;
   const u32 w0 = W * x.j;  // this lane's first word of a slab
-  if (a.n == 0) return;
-  u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
+  u64 lo = 0, hi = a.n;
+  if (a.range) {
+    hi = min(a.range[1], a.n);
+    lo = min(a.range[0], hi);
+  }
+  if (lo >= hi) return;
+  u64 i = lo + (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
   u32 next_w, next_pk, next_len, next_kind = op_find;
   auto fetch = [&](u64 e) {
-    const u64 c = e < a.n ? e : a.n - 1;  // clamped: the loads are unconditional
+    const u64 c = e < hi ? e : hi - 1;  // clamped: the loads are unconditional
     next_w = a.out_value_lengths[c];
-    next_pk = park[c];
+    next_pk = park[c - lo];
     next_len = a.key_lengths ? a.key_lengths[c] : maxK;
     if (a.types) next_kind = a.types[c];
   };
   fetch(i);
-  for (; i < a.n; i += ntiles) {
+  for (; i < hi; i += ntiles) {
     // copies made before the next entry's loads are issued into the same registers
     u32 w, pk, klen, kd = op_find;
     asm volatile("v_mov_b32 %0, %1" : "=v"(w) : "v"(next_w));

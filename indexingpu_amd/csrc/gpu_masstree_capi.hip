@@ -327,6 +327,12 @@ int gmt_set_record_tile(gmt_index* index, uint32_t record_tile) {
   return GMT_OK;
 }
 
+int gmt_reserve_find_scratch(gmt_index* index, uint64_t max_requests) {
+  if (!index) return bad("gmt_reserve_find_scratch: null handle");
+  // every tile width's instance keeps its own scratch
+  return guarded([&] { B(index)->on_all([&](auto& t) { t.reserve_find_scratch(max_requests); }); });
+}
+
 int gmt_set_mixed_split(gmt_index* index, uint32_t min_requests) {
   if (!index) return bad("gmt_set_mixed_split: null handle");
   B(index)->on_all([&](auto& t) { t.set_mixed_split(min_requests); });

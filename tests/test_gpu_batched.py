@@ -33,8 +33,10 @@ def _run_multi(g, types, keys, lens, vals, offs, mk, mv):
     return out
 
 
-@pytest.mark.parametrize("tile,mode", [(16, 0), (64, 0), (16, 1), (32, 1)])
-def test_uneven_batches_match_sequential_oracle(tile, mode):
+@pytest.mark.parametrize("tile,mode,split", [(16, 0, None), (64, 0, None), (16, 1, 0xFFFFFFFF),
+                                              (32, 1, 0xFFFFFFFF), (16, 1, 0), (32, 1, 0)])
+def test_uneven_batches_match_sequential_oracle(tile, mode, split):
+    """split 0: every batch runs its finds as a read-only pass before its other requests."""
     wl = W.MixedWorkload(50_000, key_len=4, seed=31)
     mk, mv = 4, 2
     rng = np.random.default_rng(5)
@@ -42,6 +44,8 @@ def test_uneven_batches_match_sequential_oracle(tile, mode):
     batches = [wl.batch(int(s)) for s in sizes]
     g, o = S.GpuBackend(tile, pool_bytes=128 << 20), S.OracleBackend()
     g.t.set_multi_mode(mode)
+    if split is not None:
+        g.t.set_mixed_split(split)
     g.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
     o.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
     types, keys, lens, vals, offs = _concat(batches, mk, mv)
@@ -59,8 +63,8 @@ def test_uneven_batches_match_sequential_oracle(tile, mode):
     assert g.items() == o.items()
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_batches_see_each_other_in_order(mode):
+@pytest.mark.parametrize("mode,split", [(0, None), (1, 0xFFFFFFFF), (1, 0)])
+def test_batches_see_each_other_in_order(mode, split):
     """insert K / find K / update K / find K / erase K / find K as six batches."""
     rng = np.random.default_rng(9)
     keys, lens = W.unique_random_keys(rng, 3000, 1, 5)
@@ -75,6 +79,8 @@ def test_batches_see_each_other_in_order(mode):
     types, k, l, vals, offs = _concat(batches, mk, mv)
     g = S.GpuBackend(16, pool_bytes=64 << 20)
     g.t.set_multi_mode(mode)
+    if split is not None:
+        g.t.set_mixed_split(split)
     gv, gl, gr = _run_multi(g, types, k, l, vals, offs, mk, mv)
     gv = gv.reshape(6, n, mv)
     gl = gl.reshape(6, n)
@@ -114,3 +120,41 @@ def test_tile_width_switches_between_calls():
     with pytest.raises(Exception):
         g.t.tile_size = 48
     assert g.t.tile_size == 32
+
+
+def test_batched_form_captured_into_a_graph():
+    """The batched form (one launch per batch, finds as a read-only pass) captured into a
+    graph: ranges are read on the device, grids are full and the split follows the
+    average batch size.  One replay applies the batches; outputs match the oracle."""
+    wl = W.MixedWorkload(50_000, key_len=4, seed=41)
+    mk, mv = 4, 2
+    rng = np.random.default_rng(6)
+    sizes = np.exp(rng.uniform(np.log(16), np.log(20_000), size=30)).astype(int)
+    batches = [wl.batch(int(s)) for s in sizes]
+    g, o = S.GpuBackend(16, pool_bytes=128 << 20), S.OracleBackend()
+    g.t.set_multi_mode(1)
+    g.t.set_mixed_split(0)
+    g.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
+    o.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
+    types, keys, lens, vals, offs = _concat(batches, mk, mv)
+    n = len(types)
+    dt, dk, dl = g._d(types, np.uint8), g._d(keys), g._d(lens)
+    v = g._d(vals)
+    vl = g._d(np.full(n, mv, np.uint32))
+    r = g._zeros(n, np.uint8)
+    off = torch.from_numpy(offs).cuda()
+    g.t.reserve_find_scratch(n)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        g.t.multi_batch(dt, dk, mk, dl, v, mv, vl, r, off, len(offs) - 1, n)
+    graph.replay()
+    torch.cuda.synchronize()
+    g.t.check_device_error()
+    got = (g._h(v)[:n * mv].reshape(n, mv), g._h(vl)[:n], g._h(r, np.uint8)[:n])
+    exp = [o.mixed(b["types"], b["keys"], b["lens"], b["values"],
+                   np.full(len(b["types"]), mv, np.uint32), mk, mv) for b in batches]
+    for i in range(3):
+        assert np.array_equal(got[i], np.concatenate([e[i] for e in exp]))
+    assert g.items() == o.items()
+    del graph
