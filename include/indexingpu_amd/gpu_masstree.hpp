@@ -149,6 +149,7 @@ struct gpu_masstree {
   ~gpu_masstree() {
     if (scan_stack_) (void)hipFree(scan_stack_);  // per instance, never shared
     if (park_) (void)hipFree(park_);
+    if (hints_) (void)hipFree(hints_);
     if (owner_) {
       if (err_) (void)hipFree(err_);
       if (barrier_) (void)hipFree(barrier_);
@@ -301,6 +302,8 @@ struct gpu_masstree {
       // request alone), then the other requests; stream order separates the passes.  A
       // batch writes no key twice and reads no key it writes, so the outcome equals
       // that of one launch running all requests together.
+      ensure_hints(num_requests);
+      a.hints = hints_;
       launch_finds_pass(a, stream);
       a.skip_finds = 1;
     }
@@ -310,7 +313,10 @@ struct gpu_masstree {
   // Allocates the two-pass find scratch for batches of up to max_requests requests now
   // (it otherwise grows at the first larger batch, which synchronizes; callers that
   // capture launches into a graph reserve first).
-  void reserve_find_scratch(uint64_t max_requests) { ensure_park(max_requests); }
+  void reserve_find_scratch(uint64_t max_requests) {
+    ensure_park(max_requests);
+    ensure_hints(max_requests);
+  }
 
   // Mixed batches of at least this many requests run their finds as a separate
   // read-only pass (0: every batch; UINT32_MAX: none).
@@ -370,7 +376,10 @@ struct gpu_masstree {
           largest = std::max<uint64_t>(largest, off[b + 1] - std::min(off[b], off[b + 1]));
         largest = std::min<uint64_t>(largest, total_requests);
       }
-      if (mixed_split_min_ != 0xFFFFFFFFu) ensure_park(largest);
+      if (mixed_split_min_ != 0xFFFFFFFFu) {
+        ensure_park(largest);
+        ensure_hints(largest);
+      }
       const uint32_t tiles_per_block = indexingpu_amd::kBlockThreads / tile_size;
       for (uint32_t b = 0; b < num_batches; b++) {
         indexingpu_amd::batch_args a = m.a;
@@ -384,6 +393,7 @@ struct gpu_masstree {
           split = mixed_split_min_ != 0xFFFFFFFFu && nb >= mixed_split_min_;
         }
         if (split) {
+          a.hints = hints_;
           launch_finds_pass(a, stream, nb);
           a.skip_finds = 1;
         }
@@ -748,6 +758,14 @@ struct gpu_masstree {
     IGPU_HIP_TRY(hipMalloc(&park_, n * 4));
     park_words_ = n;
   }
+  // start hints of the writes of split mixed batches (kHintWords words per request)
+  void ensure_hints(uint64_t n) {
+    if (n <= hint_requests_) return;
+    IGPU_HIP_TRY(hipDeviceSynchronize());
+    if (hints_) IGPU_HIP_TRY(hipFree(hints_));
+    IGPU_HIP_TRY(hipMalloc(&hints_, n * 4 * indexingpu_amd::kHintWords));
+    hint_requests_ = n;
+  }
 
   void ensure_scan_stack(uint64_t words) {
     if (words <= scan_stack_words_) return;
@@ -780,6 +798,8 @@ struct gpu_masstree {
   uint32_t mixed_split_min_ = 1u << 14;
   uint32_t* park_ = nullptr;
   uint64_t park_words_ = 0;
+  uint32_t* hints_ = nullptr;
+  uint64_t hint_requests_ = 0;
   uint32_t multi_grid_percent_ = 100;
   uint32_t multi_mode_ = 1;
   uint32_t* scan_stack_ = nullptr;

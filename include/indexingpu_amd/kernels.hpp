@@ -62,7 +62,14 @@ struct batch_args {
   // Mixed batches run as a read-only pass over their finds followed by a pass over the
   // rest: with skip_finds set (and types given), batch_kernel leaves find requests alone.
   u32 skip_finds;
+  // Optional start hints of write requests, kHintWords words per request counted from
+  // the launch's first request: border node, layer root, level-1 parent and layer, as
+  // the read-only pass found them (node kInvalid: none).  Written by the read-only pass
+  // of a mixed batch, read by the write pass, which starts each write at its border
+  // node instead of the root.
+  u32* hints;
 };
+constexpr u32 kHintWords = 4;
 
 __device__ __forceinline__ u64 request_index(const batch_args& a, u64 i) {
   return a.perm_mul ? (a.perm_lo + i) * a.perm_mul % a.perm_n : i;
@@ -96,10 +103,11 @@ batch_kernel(tree_view tv, batch_args a) {
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
   u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
-  u64 hi = a.n;
+  u64 lo = 0, hi = a.n;
   if (a.range) {
     hi = min(a.range[1], a.n);
-    i += min(a.range[0], hi);
+    lo = min(a.range[0], hi);
+    i += lo;
   }
   u32 ri = 0;  // request_index(a, i); batch launches hold fewer than 2^32 requests
   op_state<gpu_node> st;
@@ -129,6 +137,15 @@ batch_kernel(tree_view tv, batch_args a) {
     if (bad_len) rq.key_length = 0;  // nothing of the request is read
     c.load_key_cache(rq);
     op_begin(c, st, tv.root);
+    if (a.hints && rq.kind != op_find && !bad_len) {
+      // start at the border node the read-only pass reached (B-link: a node that split
+      // since is left by moving right; one that became interior is descended)
+      const u32* hp = a.hints + (u64)kHintWords * (ri - lo);
+      const u32 hn = hp[0], hr = hp[1], hpar = hp[2], hl = hp[3];
+      const u32 ns = tv.pool.num_slabs;
+      if (hn < ns && hr < ns && hl < rq.key_length)
+        op_begin_at(st, hn, hr, hpar < ns ? hpar : kInvalid, hl);
+    }
     if (bad_len) {
       c.set_error(kErrBadLength);
       op_finish(st, 0);

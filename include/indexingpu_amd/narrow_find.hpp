@@ -307,6 +307,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_find_kernel(tree_view tv
 // In a mixed batch both passes answer the finds only.
 constexpr u32 kPendingRecord = 0x80000000u;  // record indices stay below 2^31
 constexpr u32 kNoOutput = 0xFFFFFFFEu;       // a request the find passes do not answer
+constexpr u32 kHintDone = 0xFFFFFFFDu;       // a write request: its start hint is written
 
 // Exchanges inside a tile for the descent.  Tiles of two or four lanes use DPP quad
 // permutations (an operand modifier of a VALU instruction: no LDS traffic, no wait);
@@ -400,6 +401,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   u32 node[K], layer[K], klen[K], q[K];
   u32 next_len[K], next_q[K];  // length and first slice of lookup idx[k], loaded ahead
   u32 next_kind[K], kind[K];   // request kinds (mixed batches: only finds are looked up)
+  u32 lroot[K], par[K];        // layer root and level-1 node passed (write start hints)
   bool active[K], q_stale[K];
 
   // loads of the tile's following lookup (the index clamped into the batch, so the loads
@@ -428,6 +430,8 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       kind[k] = kd;
     }
     node[k] = tv.root;
+    lroot[k] = tv.root;
+    par[k] = kInvalid;
     layer[k] = 0;
     q_stale[k] = false;
     active[k] = idx[k] < hi;
@@ -491,26 +495,40 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       const u32 gid = x.min(mine);
       const u32 v = x.bor(mine == gid ? cand : 0u);
       const bool found = gid < nk;
+      // a write request of a mixed batch is walked to its border node when start
+      // hints are wanted (else left alone): the pass that runs the writes starts there
+      const bool write = a.types && kind[k] != op_find;
       u32 len = kInvalid;  // kInvalid: the lookup goes on
-      if (a.types && kind[k] != op_find) {
-        len = kNoOutput;  // not a find: left to the pass that runs the other requests
+      u32 hint = kInvalid;  // write: the border node reached
+      if (write && !a.hints) {
+        len = kNoOutput;
       } else if (klen[k] == 0 || klen[k] > maxK) {
-        if (x.j == 0) atomicCAS(tv.err, 0u, (u32)kErrBadLength);
-        len = 0;
+        if (write) {
+          len = kHintDone;  // the write pass reports the length
+        } else {
+          if (x.j == 0) atomicCAS(tv.err, 0u, (u32)kErrBadLength);
+          len = 0;
+        }
       } else if (!border) {
         // no bound >= q: q lies right of this node (B-link move right)
+        if (found && hdr_level(h) == 1) par[k] = node[k];
         node[k] = found ? (v & kChildMask) : sib;
       } else if (q[k] > high && sib != kInvalid) {
         node[k] = sib;
+      } else if (found && ks_get(ksb, gid) == kLink) {
+        layer[k] = l + 1;  // < key length: the entry continues
+        node[k] = v;
+        lroot[k] = v;
+        par[k] = kInvalid;
+        q_stale[k] = true;
+      } else if (write) {
+        hint = node[k];
+        len = kHintDone;
       } else if (!found) {
         len = 0;
       } else {
         const u32 ks = ks_get(ksb, gid);
-        if (ks == kLink) {
-          layer[k] = l + 1;  // < key length: the entry continues
-          node[k] = v;
-          q_stale[k] = true;
-        } else if (ks == kValue) {
+        if (ks == kValue) {
           if (x.j == 0 && maxV >= 1) a.out_values[idx[k] * (u64)maxV] = v;
           len = 1;
         } else if (v < nslabs) {
@@ -522,10 +540,17 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       }
       if (len == kInvalid && node[k] >= nslabs) {
         if (x.j == 0) atomicCAS(tv.err, 0u, (u32)kErrBadIndex);
-        len = 0;
+        len = write ? kHintDone : 0u;
       }
       if (len != kInvalid) {
-        if (x.j == 0 && len != kNoOutput) a.out_value_lengths[idx[k]] = len;
+        if (x.j == 0) {
+          if (len == kHintDone) {
+            u32* hp = a.hints + (u64)kHintWords * (idx[k] - lo);
+            *reinterpret_cast<uint4*>(hp) = make_uint4(hint, lroot[k], par[k], layer[k]);
+          } else if (len != kNoOutput) {
+            a.out_value_lengths[idx[k]] = len;
+          }
+        }
         idx[k] += stride;
         start(k);
       }

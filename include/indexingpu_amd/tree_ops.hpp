@@ -135,6 +135,19 @@ IGPU_HD void op_begin(C& c, S& st, u32 root) {
   (void)c;
 }
 
+// Starts an operation (already begun at the root) at border node `node` of layer
+// `layer`, whose root is `layer_root` and whose level-1 interior node on the way down
+// was `parent` (kInvalid: unknown).  Any node of the layer's border level left of the
+// key's is a valid start in a B-link tree: the descent moves right from it.
+template <class S>
+IGPU_HD void op_begin_at(S& st, u32 node, u32 layer_root, u32 parent, u32 layer) {
+  st.layer = layer;
+  st.node = node;
+  st.layer_root = layer_root;
+  st.hint[0] = parent;
+  st.lvl = 0;
+}
+
 template <class S>
 IGPU_HD void op_finish(S& st, u32 result) {
   st.result = result;
