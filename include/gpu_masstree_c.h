@@ -168,6 +168,14 @@ int gmt_set_record_tile(gmt_index* index, uint32_t record_tile);
  * key twice and reads no key it writes. */
 int gmt_set_mixed_split(gmt_index* index, uint32_t min_requests);
 
+/* Insert, update and erase batches of at least min_requests requests (default 1) run a
+ * read-only descent first that finds each request's border node, and start the writes
+ * there instead of at the root; 0xFFFFFFFF turns this off.  Uses 16 bytes of scratch
+ * per request for up to 4M requests at a time (allocated at the first larger batch,
+ * which synchronizes).  A tuning knob with no counterpart in the reference; results do
+ * not depend on it. */
+int gmt_set_write_hints(gmt_index* index, uint32_t min_requests);
+
 /* Allocates now the scratch that two-pass finds and split mixed batches use (4 bytes per
  * request) for batches of up to max_requests requests; without it the first larger
  * batch allocates and synchronizes.  Call it before capturing such calls into a graph.

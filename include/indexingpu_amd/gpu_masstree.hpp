@@ -228,7 +228,7 @@ struct gpu_masstree {
     a.max_value_length = max_value_length;
     a.value_lengths = value_lengths;
     a.results = results;
-    launch_batch<true>(a, stream, true);
+    launch_hinted(a, stream);
   }
 
   // Erase never merges nodes here (see tree_ops.hpp), so the merge switches of the
@@ -241,7 +241,7 @@ struct gpu_masstree {
     indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
     a.kind = indexingpu_amd::op_erase;
     a.results = results;
-    launch_batch<true>(a, stream, true);
+    launch_hinted(a, stream);
   }
 
   template <bool use_upper_key = true, bool concurrent = false, bool use_shmem_key = false>
@@ -317,6 +317,11 @@ struct gpu_masstree {
     ensure_park(max_requests);
     ensure_hints(max_requests);
   }
+
+  // Insert, update and erase batches of at least this many requests start their writes
+  // at border nodes found by a read-only pass (UINT32_MAX: never).
+  void set_write_hints(uint32_t min_requests) { write_hint_min_ = min_requests; }
+  uint32_t write_hints() const { return write_hint_min_; }
 
   // Mixed batches of at least this many requests run their finds as a separate
   // read-only pass (0: every batch; UINT32_MAX: none).
@@ -655,7 +660,7 @@ struct gpu_masstree {
     const uint64_t n = a.n;
     if (n == 0) return;
     if (std::max<uint64_t>(kGrowthMin, size * (kGrowthFactor - 1)) >= n) {
-      launch_batch<true>(a, stream, true);  // the tree is large enough: one launch
+      launch_hinted(a, stream);  // the tree is large enough: no growth stages
       *inserted_hint_ = size + n;
       return;
     }
@@ -668,11 +673,60 @@ struct gpu_masstree {
       indexingpu_amd::batch_args s = a;
       s.perm_lo = done;
       s.n = stage;
-      launch_batch<true>(s, stream, true);
+      launch_hinted(s, stream);
       done += stage;
       size += stage;
     }
     *inserted_hint_ = size;
+  }
+
+  // A write launch (insert, update or erase requests) in chunks of at most kHintChunk
+  // positions.  With write hints on, each chunk first gets a read-only descent that
+  // walks every request to its border node and leaves a start hint there, and the
+  // chunk's writes then start at those nodes instead of the root (a B-link descent may
+  // start at any node left of the key's on its level; nodes that split meanwhile are
+  // left by moving right).  The requests of a batch are independent, so chunks give
+  // the results of one launch.
+  static constexpr uint64_t kHintChunk = 1ull << 22;
+  void launch_hinted(const indexingpu_amd::batch_args& a, hipStream_t stream) {
+    const uint64_t n = a.n;
+    if (n == 0) return;
+    if (write_hint_min_ == 0xFFFFFFFFu || n < write_hint_min_) {
+      launch_batch<true>(a, stream, true);
+      return;
+    }
+    ensure_hints(std::min<uint64_t>(n, kHintChunk));
+    for (uint64_t off = 0; off < n; off += kHintChunk) {
+      indexingpu_amd::batch_args c = a;
+      c.n = std::min<uint64_t>(kHintChunk, n - off);
+      if (a.perm_mul) {
+        c.perm_lo = a.perm_lo + off;  // positions of a scattered order
+      } else if (off) {
+        c.keys = a.keys + off * a.max_key_length;
+        if (a.key_lengths) c.key_lengths = a.key_lengths + off;
+        if (a.values) c.values = a.values + off * a.max_value_length;
+        if (a.value_lengths) c.value_lengths = a.value_lengths + off;
+        if (a.results) c.results = a.results + off;
+      }
+      c.hints = hints_;
+      switch (find_tile_) {
+        case 1: launch_descend_only<1>(c, stream); break;
+        case 4: launch_descend_only<4>(c, stream); break;
+        case 8: launch_descend_only<8>(c, stream); break;
+        case 16: launch_descend_only<16>(c, stream); break;
+        default: launch_descend_only<2>(c, stream); break;
+      }
+      launch_batch<true>(c, stream, true);
+    }
+  }
+  template <uint32_t T>
+  void launch_descend_only(const indexingpu_amd::batch_args& a, hipStream_t stream) {
+    constexpr uint32_t per_block = indexingpu_amd::kBlockThreads / T;
+    auto kernel = indexingpu_amd::narrow_descend_kernel<T, 1>;
+    const uint64_t want = (a.n + per_block - 1) / per_block;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(kernel));
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
+                       view(), a, park_);
   }
 
   template <typename Kern>
@@ -796,6 +850,7 @@ struct gpu_masstree {
   uint32_t find_passes_ = 2;
   uint32_t record_tile_ = 0;
   uint32_t mixed_split_min_ = 1u << 14;
+  uint32_t write_hint_min_ = 1;
   uint32_t* park_ = nullptr;
   uint64_t park_words_ = 0;
   uint32_t* hints_ = nullptr;

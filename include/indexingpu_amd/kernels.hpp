@@ -62,11 +62,11 @@ struct batch_args {
   // Mixed batches run as a read-only pass over their finds followed by a pass over the
   // rest: with skip_finds set (and types given), batch_kernel leaves find requests alone.
   u32 skip_finds;
-  // Optional start hints of write requests, kHintWords words per request counted from
-  // the launch's first request: border node, layer root, level-1 parent and layer, as
-  // the read-only pass found them (node kInvalid: none).  Written by the read-only pass
-  // of a mixed batch, read by the write pass, which starts each write at its border
-  // node instead of the root.
+  // Optional start hints of write requests, kHintWords words per launch position
+  // (counted from the launch's first request): border node, layer root, level-1 parent
+  // and layer, as a read-only pass over the same positions found them (node kInvalid:
+  // none).  The write launch then starts each write at its border node instead of the
+  // root.
   u32* hints;
 };
 constexpr u32 kHintWords = 4;
@@ -140,7 +140,7 @@ batch_kernel(tree_view tv, batch_args a) {
     if (a.hints && rq.kind != op_find && !bad_len) {
       // start at the border node the read-only pass reached (B-link: a node that split
       // since is left by moving right; one that became interior is descended)
-      const u32* hp = a.hints + (u64)kHintWords * (ri - lo);
+      const u32* hp = a.hints + (u64)kHintWords * (i - lo);
       const u32 hn = hp[0], hr = hp[1], hpar = hp[2], hl = hp[3];
       const u32 ns = tv.pool.num_slabs;
       if (hn < ns && hr < ns && hl < rq.key_length)

@@ -399,6 +399,8 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   if (lo >= hi) return;
   u64 idx[K];
   u32 node[K], layer[K], klen[K], q[K];
+  u64 req[K], next_req[K];     // request index of lookup idx[k] (launch position), and
+                               // of the tile's next one
   u32 next_len[K], next_q[K];  // length and first slice of lookup idx[k], loaded ahead
   u32 next_kind[K], kind[K];   // request kinds (mixed batches: only finds are looked up)
   u32 lroot[K], par[K];        // layer root and level-1 node passed (write start hints)
@@ -407,7 +409,8 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   // loads of the tile's following lookup (the index clamped into the batch, so the loads
   // are unconditional and land in the registers the next start reads)
   auto fetch = [&](u32 k, u64 i) {
-    const u64 c = i < hi ? i : hi - 1;
+    const u64 c = request_index(a, i < hi ? i : hi - 1);
+    next_req[k] = c;
     next_len[k] = a.key_lengths ? a.key_lengths[c] : maxK;
     next_q[k] = a.keys[c * (u64)maxK];
     if (a.types) next_kind[k] = a.types[c];
@@ -424,6 +427,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
     asm volatile("v_mov_b32 %0, %1" : "=v"(q0) : "v"(next_q[k]));
     klen[k] = L;
     q[k] = q0;
+    req[k] = next_req[k];
     if (a.types) {
       u32 kd;
       asm volatile("v_mov_b32 %0, %1" : "=v"(kd) : "v"(next_kind[k]));
@@ -456,7 +460,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
 #pragma unroll
     for (u32 k = 0; k < K; k++) {
       if (layer[k] != 0 && q_stale[k]) {
-        q[k] = a.keys[idx[k] * (u64)maxK + layer[k]];
+        q[k] = a.keys[req[k] * (u64)maxK + layer[k]];
         q_stale[k] = false;
       }
       n[k] = load_piece<T>(pool, node[k], x.j);
@@ -497,7 +501,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       const bool found = gid < nk;
       // a write request of a mixed batch is walked to its border node when start
       // hints are wanted (else left alone): the pass that runs the writes starts there
-      const bool write = a.types && kind[k] != op_find;
+      const bool write = a.types ? kind[k] != op_find : a.kind != op_find;
       u32 len = kInvalid;  // kInvalid: the lookup goes on
       u32 hint = kInvalid;  // write: the border node reached
       if (write && !a.hints) {
@@ -529,7 +533,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       } else {
         const u32 ks = ks_get(ksb, gid);
         if (ks == kValue) {
-          if (x.j == 0 && maxV >= 1) a.out_values[idx[k] * (u64)maxV] = v;
+          if (x.j == 0 && maxV >= 1) a.out_values[req[k] * (u64)maxV] = v;
           len = 1;
         } else if (v < nslabs) {
           if (x.j == 0) park[idx[k] - lo] = (ks << 16) | l;
@@ -548,7 +552,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
             u32* hp = a.hints + (u64)kHintWords * (idx[k] - lo);
             *reinterpret_cast<uint4*>(hp) = make_uint4(hint, lroot[k], par[k], layer[k]);
           } else if (len != kNoOutput) {
-            a.out_value_lengths[idx[k]] = len;
+            a.out_value_lengths[req[k]] = len;
           }
         }
         idx[k] += stride;

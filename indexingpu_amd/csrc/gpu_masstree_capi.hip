@@ -333,6 +333,12 @@ int gmt_reserve_find_scratch(gmt_index* index, uint64_t max_requests) {
   return guarded([&] { B(index)->on_all([&](auto& t) { t.reserve_find_scratch(max_requests); }); });
 }
 
+int gmt_set_write_hints(gmt_index* index, uint32_t min_requests) {
+  if (!index) return bad("gmt_set_write_hints: null handle");
+  B(index)->on_all([&](auto& t) { t.set_write_hints(min_requests); });
+  return GMT_OK;
+}
+
 int gmt_set_mixed_split(gmt_index* index, uint32_t min_requests) {
   if (!index) return bad("gmt_set_mixed_split: null handle");
   B(index)->on_all([&](auto& t) { t.set_mixed_split(min_requests); });

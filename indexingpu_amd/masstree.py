@@ -76,6 +76,7 @@ def load_library():
     lib.gmt_set_find_passes.argtypes = [vp, u32]
     lib.gmt_set_record_tile.argtypes = [vp, u32]
     lib.gmt_set_mixed_split.argtypes = [vp, u32]
+    lib.gmt_set_write_hints.argtypes = [vp, u32]
     lib.gmt_reserve_find_scratch.argtypes = [vp, ctypes.c_uint64]
     lib.gmt_tile_size.argtypes = [vp]
     lib.gmt_tile_size.restype = u32
@@ -291,6 +292,11 @@ class GpuMasstree:
         """Mixed batches of at least min_requests requests run their finds as a separate
         read-only pass (0: every batch; 0xFFFFFFFF: none)."""
         self._check(self._lib.gmt_set_mixed_split(self._h, int(min_requests) & 0xFFFFFFFF))
+
+    def set_write_hints(self, min_requests):
+        """Insert, update and erase batches of at least min_requests requests start their
+        writes at border nodes found by a read-only descent (0xFFFFFFFF: never)."""
+        self._check(self._lib.gmt_set_write_hints(self._h, int(min_requests) & 0xFFFFFFFF))
 
     def reserve_find_scratch(self, max_requests):
         """Allocates the two-pass find scratch for batches of up to max_requests requests

@@ -13,13 +13,16 @@ def _eq(a, b):
     assert np.array_equal(np.asarray(a), np.asarray(b))
 
 
+@pytest.mark.parametrize("hints", [1, 0xFFFFFFFF], ids=["write_hints", "from_root"])
 @pytest.mark.parametrize("tile", [16, 64])
-def test_plain_batch_varlen(tile):
+def test_plain_batch_varlen(tile, hints):
     """Plain batch path shape: var-len keys of 2..8 slices, 2-slice values, one find
-    batch with 50% hits, one erase batch, then a find of everything."""
+    batch with 50% hits, one erase batch, then a find of everything; writes start at
+    border nodes found by a read-only pass, or at the root."""
     d = W.plain_batch_workload(200_000, seed=5)
     mk, mv = d["max_key_length"], d["max_value_length"]
     g, o = S.GpuBackend(tile, pool_bytes=256 << 20), S.OracleBackend()
+    g.t.set_write_hints(hints)
     _eq(g.insert(d["keys"], d["lens"], d["values"], None, mk, mv),
         o.insert(d["keys"], d["lens"], d["values"], None, mk, mv))
     for a, b in zip(g.find(d["find_keys"], d["find_lens"], mk, mv),
@@ -81,7 +84,8 @@ def test_mixed_batches_and_scans(tile, split):
     assert g.items() == o.items()
 
 
-def test_layers_push_down_and_long_keys():
+@pytest.mark.parametrize("hints", [1, 0xFFFFFFFF], ids=["write_hints", "from_root"])
+def test_layers_push_down_and_long_keys(hints):
     """Keys sharing long prefixes (trie layers, suffix push-downs) and keys longer than
     a tile (global key reads, multi-slab records), with scans across layers."""
     rng = np.random.default_rng(21)
@@ -99,6 +103,7 @@ def test_layers_push_down_and_long_keys():
     vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
     for tile in (16, 64):
         g, o = S.GpuBackend(tile, pool_bytes=512 << 20), S.OracleBackend()
+        g.t.set_write_hints(hints)
         half = n // 2
         _eq(g.insert(keys[:half], lens[:half], vals[:half], vlens[:half], mk, mv),
             o.insert(keys[:half], lens[:half], vals[:half], vlens[:half], mk, mv))
