@@ -722,7 +722,7 @@ struct gpu_masstree {
   template <uint32_t T>
   void launch_descend_only(const indexingpu_amd::batch_args& a, hipStream_t stream) {
     constexpr uint32_t per_block = indexingpu_amd::kBlockThreads / T;
-    auto kernel = indexingpu_amd::narrow_descend_kernel<T, 1>;
+    auto kernel = indexingpu_amd::narrow_descend_kernel<T, 1, true>;
     const uint64_t want = (a.n + per_block - 1) / per_block;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(kernel));
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
@@ -764,10 +764,21 @@ struct gpu_masstree {
         hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
                            view(), a, park_);
       };
-      switch (find_depth_) {
-        case 1: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 1>, tiles); break;
-        case 4: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 4>, tiles * 4); break;
-        default: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 2>, tiles * 2); break;
+      // plain find batches take the find-only kernels; the finds pass of a mixed batch
+      // (kinds, ranges, write hints) the general ones
+      const bool general = a.types || a.range || a.hints || a.perm_mul;
+      if (general) {
+        switch (find_depth_) {
+          case 1: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 1, true>, tiles); break;
+          case 4: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 4, true>, tiles * 4); break;
+          default: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 2, true>, tiles * 2); break;
+        }
+      } else {
+        switch (find_depth_) {
+          case 1: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 1>, tiles); break;
+          case 4: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 4>, tiles * 4); break;
+          default: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 2>, tiles * 2); break;
+        }
       }
       switch (record_tile_ ? record_tile_ : T) {
         case 1: launch_record<1>(a, stream, work); break;

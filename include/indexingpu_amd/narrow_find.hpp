@@ -377,7 +377,13 @@ struct tile_xchg {
 // and the owner's child/value with two exchanges.  Slots at or past the node's key count
 // hold no entry, but every such slot lies above every real entry, so the tile's lowest
 // qualifying slot is a real entry exactly when it is below the key count.
-template <u32 T, u32 K>
+//
+// General = false: a read-only find batch (every request a find, in array order).
+// General = true adds what the passes over mixed and write batches need: request kinds
+// (only finds are answered), the device-read range of the batched form, a scattered
+// request order (request_index), and the start hints of write requests.  The two are
+// separate instantiations, so find batches carry none of it.
+template <u32 T, u32 K, bool General = false>
 __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view tv, batch_args a,
                                                                         u32* park) {
   constexpr u32 S = kSlots / T;  // slots per lane
@@ -390,30 +396,38 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   const u64 first = ((u64)blockIdx.x * tiles_per_block + threadIdx.x / T) * K;
   const u32 maxK = a.max_key_length, maxV = a.max_value_length;
   const u32 slot0 = S * x.j;  // this lane's first slot
-  // requests [lo, hi): the batch, or the range read on the device (batched form)
+  // launch positions [lo, hi): the batch, or the range read on the device (batched form)
   u64 lo = 0, hi = a.n;
-  if (a.range) {
-    hi = min(a.range[1], a.n);
-    lo = min(a.range[0], hi);
+  if constexpr (General) {
+    if (a.range) {
+      hi = min(a.range[1], a.n);
+      lo = min(a.range[0], hi);
+    }
   }
   if (lo >= hi) return;
-  u64 idx[K];
+  u64 idx[K];                  // launch position of lookup k
   u32 node[K], layer[K], klen[K], q[K];
-  u64 req[K], next_req[K];     // request index of lookup idx[k] (launch position), and
-                               // of the tile's next one
+  u64 req[K], next_req[K];     // (General) request index of lookup k, and of the next
   u32 next_len[K], next_q[K];  // length and first slice of lookup idx[k], loaded ahead
-  u32 next_kind[K], kind[K];   // request kinds (mixed batches: only finds are looked up)
-  u32 lroot[K], par[K];        // layer root and level-1 node passed (write start hints)
+  u32 next_kind[K], kind[K];   // (General) request kinds: only finds are answered
+  u32 lroot[K], par[K];        // (General) layer root and level-1 node passed (hints)
   bool active[K], q_stale[K];
 
-  // loads of the tile's following lookup (the index clamped into the batch, so the loads
-  // are unconditional and land in the registers the next start reads)
+  auto request = [&](u32 k) -> u64 {
+    if constexpr (General) return req[k];
+    else return idx[k];
+  };
+  // loads of the tile's following lookup (the position clamped into the batch, so the
+  // loads are unconditional and land in the registers the next start reads)
   auto fetch = [&](u32 k, u64 i) {
-    const u64 c = request_index(a, i < hi ? i : hi - 1);
-    next_req[k] = c;
+    u64 c = i < hi ? i : hi - 1;
+    if constexpr (General) {
+      c = request_index(a, c);
+      next_req[k] = c;
+      if (a.types) next_kind[k] = a.types[c];
+    }
     next_len[k] = a.key_lengths ? a.key_lengths[c] : maxK;
     next_q[k] = a.keys[c * (u64)maxK];
-    if (a.types) next_kind[k] = a.types[c];
   };
   // begins lookup idx[k] (its length and first slice are in next_len/next_q, loaded one
   // lookup ahead) and loads those of the tile's following lookup, which arrive while
@@ -427,15 +441,17 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
     asm volatile("v_mov_b32 %0, %1" : "=v"(q0) : "v"(next_q[k]));
     klen[k] = L;
     q[k] = q0;
-    req[k] = next_req[k];
-    if (a.types) {
-      u32 kd;
-      asm volatile("v_mov_b32 %0, %1" : "=v"(kd) : "v"(next_kind[k]));
-      kind[k] = kd;
+    if constexpr (General) {
+      req[k] = next_req[k];
+      if (a.types) {
+        u32 kd;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(kd) : "v"(next_kind[k]));
+        kind[k] = kd;
+      }
+      lroot[k] = tv.root;
+      par[k] = kInvalid;
     }
     node[k] = tv.root;
-    lroot[k] = tv.root;
-    par[k] = kInvalid;
     layer[k] = 0;
     q_stale[k] = false;
     active[k] = idx[k] < hi;
@@ -460,7 +476,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
 #pragma unroll
     for (u32 k = 0; k < K; k++) {
       if (layer[k] != 0 && q_stale[k]) {
-        q[k] = a.keys[req[k] * (u64)maxK + layer[k]];
+        q[k] = a.keys[request(k) * (u64)maxK + layer[k]];
         q_stale[k] = false;
       }
       n[k] = load_piece<T>(pool, node[k], x.j);
@@ -499,10 +515,11 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       const u32 gid = x.min(mine);
       const u32 v = x.bor(mine == gid ? cand : 0u);
       const bool found = gid < nk;
-      // a write request of a mixed batch is walked to its border node when start
-      // hints are wanted (else left alone): the pass that runs the writes starts there
-      const bool write = a.types ? kind[k] != op_find : a.kind != op_find;
-      u32 len = kInvalid;  // kInvalid: the lookup goes on
+      // (General) a write request is walked to its border node when start hints are
+      // wanted, else left alone: the pass that runs the writes starts there
+      bool write = false;
+      if constexpr (General) write = a.types ? kind[k] != op_find : a.kind != op_find;
+      u32 len = kInvalid;   // kInvalid: the lookup goes on
       u32 hint = kInvalid;  // write: the border node reached
       if (write && !a.hints) {
         len = kNoOutput;
@@ -515,15 +532,19 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
         }
       } else if (!border) {
         // no bound >= q: q lies right of this node (B-link move right)
-        if (found && hdr_level(h) == 1) par[k] = node[k];
+        if constexpr (General) {
+          if (found && hdr_level(h) == 1) par[k] = node[k];
+        }
         node[k] = found ? (v & kChildMask) : sib;
       } else if (q[k] > high && sib != kInvalid) {
         node[k] = sib;
       } else if (found && ks_get(ksb, gid) == kLink) {
         layer[k] = l + 1;  // < key length: the entry continues
         node[k] = v;
-        lroot[k] = v;
-        par[k] = kInvalid;
+        if constexpr (General) {
+          lroot[k] = v;
+          par[k] = kInvalid;
+        }
         q_stale[k] = true;
       } else if (write) {
         hint = node[k];
@@ -533,7 +554,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       } else {
         const u32 ks = ks_get(ksb, gid);
         if (ks == kValue) {
-          if (x.j == 0 && maxV >= 1) a.out_values[req[k] * (u64)maxV] = v;
+          if (x.j == 0 && maxV >= 1) a.out_values[request(k) * (u64)maxV] = v;
           len = 1;
         } else if (v < nslabs) {
           if (x.j == 0) park[idx[k] - lo] = (ks << 16) | l;
@@ -548,11 +569,15 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       }
       if (len != kInvalid) {
         if (x.j == 0) {
-          if (len == kHintDone) {
-            u32* hp = a.hints + (u64)kHintWords * (idx[k] - lo);
-            *reinterpret_cast<uint4*>(hp) = make_uint4(hint, lroot[k], par[k], layer[k]);
-          } else if (len != kNoOutput) {
-            a.out_value_lengths[req[k]] = len;
+          if constexpr (General) {
+            if (len == kHintDone) {
+              u32* hp = a.hints + (u64)kHintWords * (idx[k] - lo);
+              *reinterpret_cast<uint4*>(hp) = make_uint4(hint, lroot[k], par[k], layer[k]);
+            } else if (len != kNoOutput) {
+              a.out_value_lengths[request(k)] = len;
+            }
+          } else {
+            a.out_value_lengths[idx[k]] = len;
           }
         }
         idx[k] += stride;
