@@ -382,9 +382,11 @@ def run_served(torch, args):
     """Served form against the plain path on one resident index: args.served_keys keys
     of 2..8 slices (2-slice values), traces of requests of floor(exp(U(0, ln 4096)))
     operations (80% find / 10% insert / 5% erase / 5% scan, limit 50) with Poisson
-    arrivals at a sweep of rates.  The served loop (continuous batching, waves on 4
-    streams) and the plain path (one mixed-batch call plus one scan call per request,
-    in order) replay traces of the same kind; each point reports Mops/s (operations of
+    arrivals at a sweep of rates.  The served loop (continuous batching: waves side by side
+    on 4 streams, one validated launch each; or, as served_ordered, the point ops of all
+    waves in order on one stream as a read-only pass then the writes) and the plain path
+    (one mixed-batch call plus one scan call per request, in order) replay traces of the
+    same kind; each point reports Mops/s (operations of
     completed requests over the run's wall time) and p50/p99/p99.9 latency from arrival
     to completion.  Point-op results are checked against the generator's expectation;
     scan counts are checked (every scan lies in a region with >= 50 keys after it, except
@@ -405,17 +407,18 @@ def run_served(torch, args):
     tree.check_device_error()
     log(f"[served] preload {pre:.1f} Mops/s")
     rates = [float(x) for x in args.served_rates.split(",")]
-    points = {"served": [], "plain": []}
-    for mode in ("served", "plain"):
+    modes = [m for m in args.served_modes.split(",") if m]
+    points = {m: [] for m in modes}
+    for mode in modes:
         sv = MasstreeServer(tree, streams=4, max_wave_ops=args.served_wave_ops,
                             max_wave_scans=4096, ring_rows=1 << 22, epoch_waves=16,
-                            coalesce=(mode == "served"))
+                            coalesce=(mode != "plain"), ordered=(mode == "served_ordered"))
         for rate in rates:
             nreq = int(min(args.served_max_requests, max(2000, rate * args.served_seconds)))
             arena, req, expect = wl.trace(nreq, rate)
             mc = arena["scan_max_count"]
             n_scans = int(req["scan_offset"][-1] + req["num_scans"][-1])
-            arena = D.ServeWorkload.with_ring(arena, (1 << 22) if mode == "served" else n_scans * mc)
+            arena = D.ServeWorkload.with_ring(arena, (1 << 22) if mode != "plain" else n_scans * mc)
             out, st = sv.run(arena, req)
             tree.check_device_error()
             ok = all(torch.equal(arena[key].view(-1), expect[key].reshape(-1).to(arena[key].dtype))
@@ -442,9 +445,11 @@ def run_served(torch, args):
     def best(pts):
         good = [p for p in pts if p["p99_us"] <= 1000 and p["verified"]]
         return max(good, key=lambda p: p["mops"]) if good else None
-    bs, bp = best(points["served"]), best(points["plain"])
+    bs, bp = best(points.get("served", [])), best(points.get("plain", []))
+    bso = best(points.get("served_ordered", []))
     return dict(served=dict(resident_keys=n, preload_mops=round(pre, 2), points=points,
-                            best_served_at_p99_1ms=bs, best_plain_at_p99_1ms=bp,
+                            best_served_at_p99_1ms=bs, best_ordered_at_p99_1ms=bso,
+                            best_plain_at_p99_1ms=bp,
                             ratio=(round(bs["mops"] / bp["mops"], 2) if bs and bp else None)))
 
 
@@ -478,10 +483,16 @@ def main():
                     help="also run the served form against the plain path (rate sweep)")
     ap.add_argument("--served-keys", type=int, default=1 << 29)
     ap.add_argument("--served-pool-gb", type=float, default=112.0)
-    ap.add_argument("--served-rates", default="1000,3000,10000,30000,100000,300000,600000,1000000,2000000")
+    ap.add_argument("--served-rates",
+                    default="1000,3000,10000,30000,100000,300000,600000,800000,1000000,2000000")
     ap.add_argument("--served-seconds", type=float, default=0.25)
     ap.add_argument("--served-max-requests", type=int, default=1_000_000)
     ap.add_argument("--served-wave-ops", type=int, default=1 << 18)
+    ap.add_argument("--served-modes", default="served,served_ordered,plain",
+                    help="served (waves side by side on 4 streams, one validated launch "
+                         "each), served_ordered (point ops of all waves in order on one "
+                         "stream, read-only passes), plain (one mixed-batch + scan call "
+                         "per request)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a kernel of "
                          "known traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")

@@ -194,10 +194,13 @@ uint32_t gmt_tile_size(gmt_index* index);
  * No counterpart in the reference (its batch calls are the whole interface; it has
  * no request loop).  A server replays a trace of requests against the index in real
  * time with continuous batching: requests are admitted in arrival order under a
- * per-wave op/scan limit, a free-slab budget and a scan-output ring budget; each wave
- * is one mixed-batch launch plus one scan launch on one of `streams` HIP streams, so
- * waves overlap on the device.  coalesce = 0 runs the plain path instead (one request
- * per gmt_mixed_batch / gmt_scan pair, in order, on one stream) for comparison. */
+ * per-wave op/scan limit, a free-slab budget and a scan-output ring budget.  With
+ * ordered = 0 each wave is one mixed-batch launch plus one scan launch on one of
+ * `streams` HIP streams, so waves overlap on the device; with ordered = 1 the point ops
+ * of up to `streams` waves queue in order on one stream, each wave a read-only pass and
+ * then its writes, and its scans run beside them on the wave's own stream.  coalesce = 0 runs the plain path instead
+ * (one request per gmt_mixed_batch / gmt_scan pair, in order, on one stream) for
+ * comparison. */
 typedef struct gmt_server gmt_server;
 
 typedef struct {
@@ -208,6 +211,10 @@ typedef struct {
   uint64_t ring_rows;        /* scan result ring, rows of scan_max_count results */
   uint32_t epoch_waves;      /* waves between retire-list flips */
   uint32_t coalesce;         /* 1: served form; 0: plain path */
+  uint32_t ordered;          /* 1: the point ops of all waves queued in order on one stream,
+                                each wave a read-only pass (finds, write start hints) then
+                                its writes, its scans beside them on the wave's stream;
+                                0 (recommended): waves side by side on `streams` streams */
 } gmt_serve_config;
 
 /* Device arena of a trace: point ops in gmt_mixed_batch's layout, scans in gmt_scan's

@@ -453,17 +453,6 @@ __device__ __forceinline__ int rec_tail_cmp(wave_tile<T, C>& c, const u32* key, 
   return tl < len ? -1 : (tl > len ? 1 : 0);
 }
 
-// writes the output key for an entry: stack prefix (layers 0..l-1), slice, record tail
-template <u32 T, bool C>
-__device__ __forceinline__ void scan_write_key_prefix(wave_tile<T, C>& c, const scan_query<T>& q,
-                                                      u32 out, u32 l, u32 slice) {
-  if (!q.keys) return;
-  u32* dst = q.keys + (u64)out * q.maxK;
-  for (u32 j = c.lane; j <= l && j < q.maxK; j += T) {
-    dst[j] = (j == l) ? slice : q.stack[3 * j];
-  }
-}
-
 template <u32 T, bool Validate, bool C>
 __device__ __forceinline__ void scan_read(wave_tile<T, C>& c, u32 idx, gpu_node& n) {
   for (u32 spin = 0; spin < (1u << 20); spin++) {
@@ -549,29 +538,64 @@ __device__ u32 tile_scan(wave_tile<T, C>& c, u32 root, const scan_query<T>& q) {
           }
         }
       }
-      // emit entries [p, out_end), at most limit - count
+      // emit entries [p, out_end), at most limit - count.  Lane e of the tile (e < 16)
+      // emits entry e itself from the first 64 bytes of its record, so the records of a
+      // node's entries are read at once; a record of more than 16 words is emitted by the
+      // whole tile afterwards, one at a time.
       u32 take = out_end > p ? out_end - p : 0u;
       if (take > q.limit - count) take = q.limit - count;
-      for (u32 e = p; e < p + take; e++) {
-        const u32 o = count + (e - p);
-        const u32 slice = c.bcast(slot_lo(n.s), e);
-        const u32 val = c.bcast(slot_hi(n.s), e);
+      {
+        const u32 e = c.slot;
+        const bool mine = c.lane < 16 && e >= p && e < p + take;
+        const u32 slice = slot_lo(n.s), val = slot_hi(n.s);
         const u32 ks = ks_get(ksb, e);
-        scan_write_key_prefix(c, q, o, l, slice);
-        if (ks == kValue) {
-          if (c.lane == 0) {
+        if (c.ballot(mine && ks != kValue && val >= c.pool.num_slabs)) {
+          c.set_error(kErrBadIndex);
+          c.fatal = true;
+          return count;
+        }
+        bool longrec = false;
+        if (mine) {
+          const u32 o = count + (e - p);
+          if (q.keys) {
+            u32* dst = q.keys + (u64)o * q.maxK;
+            for (u32 j = 0; j <= l && j < q.maxK; j++) dst[j] = (j == l) ? slice : q.stack[3 * j];
+          }
+          if (ks == kValue) {
             if (q.vlens) q.vlens[o] = 1;
             if (q.klens) q.klens[o] = l + 1;
             if (q.vals && q.maxV >= 1) q.vals[(u64)o * q.maxV] = val;
-          }
-        } else {
-          u32 lw;
-          if (!c.rec_load(val, lw)) return count;
-          const u32 tl = rec_tail_len(lw), vl = rec_value_len(lw);
-          if (c.lane == 0) {
+          } else {
+            u64 w[8];
+#pragma unroll
+            for (u32 i = 0; i < 8; i++) w[i] = c.ld(c.slab(val) + i);
+            const u32 lw = slot_lo(w[0]);
+            const u32 tl = rec_tail_len(lw), vl = rec_value_len(lw);
+            const u32 total = 1 + tl + vl;
             if (q.vlens) q.vlens[o] = vl;
             if (q.klens) q.klens[o] = l + 1 + tl;
+            if (total <= 16) {
+#pragma unroll
+              for (u32 d = 1; d < 16; d++) {
+                const u32 word = (d & 1) ? slot_hi(w[d / 2]) : slot_lo(w[d / 2]);
+                if (d >= total) continue;
+                if (d <= tl) {
+                  if (q.keys && l + d < q.maxK) q.keys[(u64)o * q.maxK + l + d] = word;
+                } else if (q.vals && d - 1 - tl < q.maxV) {
+                  q.vals[(u64)o * q.maxV + d - 1 - tl] = word;
+                }
+              }
+            } else {
+              longrec = true;
+            }
           }
+        }
+        for (u32 lm = c.ballot(longrec); lm; lm &= lm - 1) {
+          const u32 e2 = (u32)__builtin_ctz(lm);
+          const u32 o = count + (e2 - p);
+          u32 lw;
+          if (!c.rec_load(c.bcast(slot_hi(n.s), e2), lw)) return count;
+          const u32 tl = rec_tail_len(lw), vl = rec_value_len(lw);
           // record words: tail -> key slices l+1.., value -> value slots
           const u32 total = 1 + tl + vl;
           u64 cur = c.rec_slot;

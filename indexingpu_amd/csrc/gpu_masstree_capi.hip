@@ -388,6 +388,7 @@ int gmt_server_create(gmt_index* index, const gmt_serve_config* config, gmt_serv
   c.ring_rows = config->ring_rows;
   c.epoch_waves = config->epoch_waves;
   c.coalesce = config->coalesce;
+  c.ordered = config->ordered;
   gmt_server* sv = new gmt_server();
   gmt_impl* p = B(index);
   int rc = guarded([&] {

@@ -49,8 +49,9 @@ def _scan_rows_equal(a_mix, out_mix, a_alone, req, mc):
             assert torch.equal(m[2][b * mv:(b + c) * mv], p[2][b * mv:(b + c) * mv])
 
 
+@pytest.mark.parametrize("ordered", [1, 0], ids=["ordered_waves", "side_by_side"])
 @pytest.mark.parametrize("streams,wave_ops,epoch", [(4, 4096, 3), (2, 1 << 16, 1)])
-def test_same_result_alone_and_in_the_mix(streams, wave_ops, epoch):
+def test_same_result_alone_and_in_the_mix(streams, wave_ops, epoch, ordered):
     wl = D.ServeWorkload(N_PRELOAD, seed=11)
     arena, req, expect = wl.trace(1500, rate=3e6)  # arrivals far faster than service
     mc = arena["scan_max_count"]
@@ -61,7 +62,7 @@ def test_same_result_alone_and_in_the_mix(streams, wave_ops, epoch):
 
     ga, gb = _loaded(wl), _loaded(wl)
     sv = MasstreeServer(ga.t, streams=streams, max_wave_ops=wave_ops, max_wave_scans=256,
-                        ring_rows=n_scans * mc, epoch_waves=epoch)
+                        ring_rows=n_scans * mc, epoch_waves=epoch, ordered=ordered)
     out_mix, st = sv.run(mix, req)
     ga.t.check_device_error()
     assert st["device_error"] == 0
@@ -84,7 +85,8 @@ def test_same_result_alone_and_in_the_mix(streams, wave_ops, epoch):
     assert lat["p50_us"] > 0
 
 
-def test_scans_match_oracle_and_ring_backpressure():
+@pytest.mark.parametrize("ordered", [1, 0], ids=["ordered_waves", "side_by_side"])
+def test_scans_match_oracle_and_ring_backpressure(ordered):
     """A ring that holds one full wave's scans: admission waits for rows to free,
     results still match the oracle replaying the read region."""
     wl = D.ServeWorkload(60_000, seed=12)
@@ -96,7 +98,7 @@ def test_scans_match_oracle_and_ring_backpressure():
     o.insert(k, l, v, None, wl.mk, wl.mv)
     a = D.ServeWorkload.with_ring(D.ServeWorkload.fresh_outputs(arena), 256 * mc)
     sv = MasstreeServer(g.t, streams=4, max_wave_ops=4096, max_wave_scans=256,
-                        ring_rows=256 * mc, epoch_waves=2)
+                        ring_rows=256 * mc, epoch_waves=2, ordered=ordered)
     out, st = sv.run(a, req)
     g.t.check_device_error()
     assert st["admission_waits"] > 0
