@@ -111,9 +111,12 @@ int gmt_multi_batch(gmt_index* index, const uint8_t* types, const uint32_t* keys
 
 /* Tuning knobs of gmt_multi_batch: share (1..100 %) of the resident-block capacity its
  * launches use, and how it runs: mode 0 = one persistent launch with a grid barrier
- * between batches; mode 1 (default) = one launch per batch, each reading its range from
- * `offsets` on the device, with one reclamation pass at the end (total_requests must be
- * below 2^32, else mode 0 is used).  Results do not depend on either knob. */
+ * between batches; mode 1 (default) = per batch, a read-only pass over its finds when it
+ * holds at least the gmt_set_mixed_split size, one launch for the rest, each reading its
+ * range from `offsets` on the device, and a reclamation pass (total_requests must be
+ * below 2^32, else mode 0 is used).  Outside graph capture mode 1 reads `offsets` back
+ * once (a synchronization) to size each batch's launches.  Results do not depend on
+ * either knob. */
 int gmt_set_multi_grid_percent(gmt_index* index, uint32_t percent);
 int gmt_set_multi_mode(gmt_index* index, uint32_t mode);
 
