@@ -598,23 +598,26 @@ def main():
     abytes = find_bytes(d["find_lens"], n // 2, mv)
     achieved = abytes / (mean_ms / 1e3) / 1e9
 
-    # HBM traffic per launch of this kernel on this workload, from the committed
-    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated by a 1 GiB copy); null when the
-    # tuned kernel or the workload differ from the profiled ones
+    # HBM traffic per find batch of the tuned kernels on this workload, from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated by a 1 GiB copy; one
+    # entry per kernel, every configuration the tuner tries); null for another workload
     traffic, traffic_note = None, None
     tpath = os.path.join(ROOT, "profiles", "find_kernel_16M_varlen_traffic.json")
     if best.get("find_passes", 1) == 2:
         rtile = best.get("record_tile", 0) or best["find_tile"]
-        kname = (f"narrow_descend_kernel<{best['find_tile']}, {best['find_depth']}> + "
-                 f"narrow_record_kernel<{rtile}>")
+        parts = [f"narrow_descend_kernel<{best['find_tile']}, {best['find_depth']}>",
+                 f"narrow_record_kernel<{rtile}>"]
     else:
-        kname = f"narrow_find_kernel<{best['find_tile']}, {best['find_depth']}>"
-    if os.path.exists(tpath):
+        parts = [f"narrow_find_kernel<{best['find_tile']}, {best['find_depth']}>"]
+    kname = " + ".join(parts)
+    if os.path.exists(tpath) and n == (1 << 24) and args.seed == 2:
         t = json.load(open(tpath))
-        if t["kernel"] == kname and n == (1 << 24) and args.seed == 2:
-            traffic = t["traffic_bytes_per_launch"]
-            traffic_note = ("profiles/find_kernel_16M_varlen_pmc.md: FETCH_SIZE x 1024 x "
-                            f"{t['read_scale']} + WRITE_SIZE x 1024 x {t['write_scale']}")
+        ks = t.get("kernels", {})
+        if all(p in ks for p in parts):
+            traffic = sum(ks[p]["bytes"] for p in parts)
+            traffic_note = ("profiles/find_kernel_16M_varlen_traffic.json: FETCH_SIZE x 1024 x "
+                            f"{t['read_scale']} + WRITE_SIZE x 1024 x {t['write_scale']}, "
+                            "summed over the kernels of a find batch")
     result = {
         "metric": METRIC,
         "value": round(value, 2),
