@@ -141,14 +141,15 @@ struct wave_tile {
   }
   // Locks node idx only if its header still equals `expect` (the unlocked header of a
   // validated read): kLockOk, kLockBusy (locked by another tile) or kLockStale (the
-  // node changed: re-read it).  The compare-and-swap is attempted only when a plain
-  // coherent load shows the expected header, so waiters do not queue atomics.
-  __device__ __forceinline__ u32 try_lock_snapshot(u32 idx, u32 expect) {
+  // node changed: re-read it).  On a retry the compare-and-swap is attempted only when
+  // a plain coherent load shows the expected header, so waiters do not queue atomics;
+  // `direct` (the header was validated by the step just before) skips that load.
+  __device__ __forceinline__ u32 try_lock_snapshot(u32 idx, u32 expect, bool direct = false) {
     if (!check(idx)) return kLockStale;
     u32 r = kLockStale;
     if (lane == 0) {
       u32* hp = hdr_ptr(idx);
-      u32 h = dev::coherent_load(hp);
+      u32 h = direct ? expect : dev::coherent_load(hp);
       if (h == expect) {
         u32 e = expect;
         if (__hip_atomic_compare_exchange_strong(hp, &e, expect | kLockBit, __ATOMIC_RELAXED,

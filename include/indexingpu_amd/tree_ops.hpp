@@ -79,6 +79,7 @@ struct op_state {
   u32 t0;          // coarse clock when the time guard started (0 = not yet)
   u32 hint[4];     // last interior node visited at levels 1..4 of the current layer
   u32 lvl;         // expected level of `node` (kInvalid: unknown)
+  u32 fresh;       // 1: `snap_hdr` was validated by the step just before this lock attempt
   NodeImage img;   // last validated image of `node`
 };
 
@@ -130,6 +131,7 @@ IGPU_HD void op_begin(C& c, S& st, u32 root) {
   st.result = 0;
   st.steps = 0;
   st.t0 = 0;
+  st.fresh = 0;
   clear_hints(st);
   st.phase = ph_descend;
   (void)c;
@@ -257,6 +259,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         return true;
       }
       st.snap_hdr = h;
+      st.fresh = 1;
       st.phase = ph_lock;
       return true;
     }
@@ -287,7 +290,10 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       // backoff like a busy lock: retrying at once would have every waiter re-read and
       // race for the lock after each release of a hot node (a batch whose keys all land
       // past the current right edge of the tree ran 5-10x slower that way).
-      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr);
+      // the first attempt right after the validating read goes straight to the
+      // compare-and-swap; a retry first checks the header with a plain load
+      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr, st.fresh != 0);
+      st.fresh = 0;
       if (r == kLockBusy) return false;
       if (r == kLockStale) {
         st.phase = ph_descend;
@@ -495,6 +501,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       }
       if (lvl == st.a3) {
         st.snap_hdr = h;
+        st.fresh = 1;
         st.phase = ph_parent_lock;
         return true;
       }
@@ -506,7 +513,8 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
     }
 
     case ph_parent_lock: {
-      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr);
+      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr, st.fresh != 0);
+      st.fresh = 0;
       if (r == kLockBusy) return false;
       if (r == kLockStale) {  // re-read the candidate (it may have split or grown)
         st.phase = ph_parent_descend;

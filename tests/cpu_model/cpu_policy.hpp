@@ -95,9 +95,9 @@ struct cpu_tile {
     __atomic_thread_fence(__ATOMIC_RELEASE);
     return true;
   }
-  u32 try_lock_snapshot(u32 idx, u32 expect) {
+  u32 try_lock_snapshot(u32 idx, u32 expect, bool direct = false) {
     if (!check(idx)) return kLockStale;
-    u32 h = __atomic_load_n(hdr_ptr(idx), __ATOMIC_ACQUIRE);
+    u32 h = direct ? expect : __atomic_load_n(hdr_ptr(idx), __ATOMIC_ACQUIRE);
     if (h == expect) {
       u32 e = expect;
       if (__atomic_compare_exchange_n(hdr_ptr(idx), &e, expect | kLockBit, false,
