@@ -23,6 +23,7 @@ namespace {
 struct model {
   shared_pool pool;
   u32 root;
+  u32 hinted = 0;  // writes start at hints from a read-only walk (as the GPU write passes)
   explicit model(u32 n) : pool(n) {
     cpu_tile t(pool);
     root = t.alloc_slab();
@@ -74,6 +75,76 @@ void store_result(const batch_args& a, const request& rq, uint64_t i, uint32_t r
 }
 
 using state_t = op_state<cpu_model::node_img>;
+
+// The start hint of a write, as the GPU's read-only pass (narrow_descend_kernel<T, K, true>)
+// leaves it before the write pass: the border node the key reaches with no writer running,
+// the root of that layer, the level-1 node passed on the way (kInvalid: none) and the layer.
+struct start_hint {
+  u32 node = kInvalid, root = kInvalid, par = kInvalid, layer = 0;
+};
+
+start_hint read_only_hint(cpu_tile& c, u32 root, const request& rq) {
+  start_hint h;
+  if (rq.key_length == 0) return h;
+  u32 node = root, lroot = root, par = kInvalid, layer = 0;
+  for (u32 guard = 0; guard < (1u << 24); guard++) {
+    cpu_model::node_img n;
+    if (!c.read_node<false>(node, n)) return h;
+    const u32 hd = c.header(n);
+    const u32 q = rq.key[layer];
+    if (!hdr_border(hd)) {
+      const u32 child = c.find_child(n, q);
+      if (child != kInvalid && hdr_level(hd) == 1) par = node;
+      node = child != kInvalid ? child : c.sibling(n);
+      if (node == kInvalid) return h;
+      continue;
+    }
+    if (q > c.high(n) && c.sibling(n) != kInvalid) {
+      node = c.sibling(n);
+      continue;
+    }
+    u32 ks = 0, val = 0;
+    const int pos = c.lookup(n, q, layer + 1 < rq.key_length, ks, val);
+    if (pos >= 0 && ks == kLink) {
+      layer += 1;
+      if (layer >= rq.key_length) return h;
+      node = lroot = val;
+      par = kInvalid;
+      continue;
+    }
+    h.node = node;
+    h.root = lroot;
+    h.par = par;
+    h.layer = layer;
+    return h;
+  }
+  return h;
+}
+
+// hints of every write request of the batch (computed before any write runs)
+std::vector<start_hint> batch_hints(model& M, const batch_args& a, uint64_t n) {
+  std::vector<start_hint> hs(M.hinted ? n : 0);
+  cpu_tile c(M.pool);
+  uint32_t scratch = 0;
+  for (uint64_t i = 0; i < hs.size(); i++) {
+    request r = make_request(a, i, &scratch);
+    if (r.kind != op_find) hs[i] = read_only_hint(c, M.root, r);
+  }
+  return hs;
+}
+
+// op_begin, then the hinted start of a write when there is a hint
+void begin_request(model& M, cpu_tile& c, state_t& s, const request& r,
+                   const std::vector<start_hint>& hs, uint64_t i) {
+  op_begin(c, s, M.root);
+  if (r.key_length == 0) {
+    op_finish(s, 0);
+    return;
+  }
+  if (i < hs.size() && r.kind != op_find && hs[i].node != kInvalid &&
+      hs[i].layer < r.key_length)
+    op_begin_at(s, hs[i].node, hs[i].root, hs[i].par, hs[i].layer);
+}
 }  // namespace
 
 extern "C" {
@@ -83,6 +154,7 @@ void cm_destroy(void* h) { delete static_cast<model*>(h); }
 uint32_t cm_error(void* h) { return static_cast<model*>(h)->pool.error.load(); }
 uint32_t cm_slabs_used(void* h) { return static_cast<model*>(h)->pool.next.load(); }
 uint64_t cm_retired(void* h) { return static_cast<model*>(h)->pool.retired.size(); }
+void cm_set_hinted(void* h, uint32_t on) { static_cast<model*>(h)->hinted = on; }
 
 // Runs one batch.  nthreads == 0: interleaved simulation with ntiles tiles and the
 // given scheduler seed; nthreads > 0: that many host threads (static partition).
@@ -94,6 +166,7 @@ void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32
   model& M = *static_cast<model*>(h);
   batch_args a{kinds, default_kind, keys, maxK, klens, values, maxV, vlens,
                out_values, out_vlens, results};
+  const std::vector<start_hint> hs = batch_hints(M, a, n);
   if (nthreads == 0) {
     std::mt19937_64 rng(seed);
     std::vector<state_t> st(ntiles);
@@ -108,8 +181,7 @@ void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32
       cur[t] = t;
       if (t < n) {
         rq[t] = make_request(a, t, &scratch[t]);
-        op_begin(ctx[t], st[t], M.root);
-        if (rq[t].key_length == 0) op_finish(st[t], 0);
+        begin_request(M, ctx[t], st[t], rq[t], hs, t);
         active.push_back(t);
       }
     }
@@ -122,8 +194,7 @@ void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32
         cur[t] += ntiles;
         if (cur[t] < n) {
           rq[t] = make_request(a, cur[t], &scratch[t]);
-          op_begin(ctx[t], st[t], M.root);
-          if (rq[t].key_length == 0) op_finish(st[t], 0);
+          begin_request(M, ctx[t], st[t], rq[t], hs, cur[t]);
         } else {
           active[pick] = active.back();
           active.pop_back();
@@ -140,8 +211,7 @@ void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32
       uint32_t scratch = 0;
       for (uint64_t i = t; i < n; i += nthreads) {
         request r = make_request(a, i, &scratch);
-        op_begin(c, s, M.root);
-        if (r.key_length == 0) op_finish(s, 0);
+        begin_request(M, c, s, r, hs, i);
         while (s.phase != ph_done) {
           if (!op_step<true>(c, r, s)) std::this_thread::yield();
         }

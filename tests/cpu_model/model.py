@@ -27,6 +27,7 @@ def _load():
         lib.cm_slabs_used.restype = u32
         lib.cm_retired.argtypes = [vp]
         lib.cm_retired.restype = u64
+        lib.cm_set_hinted.argtypes = [vp, u32]
         lib.cm_batch.argtypes = [vp, u8p, u32, u32p, u32, u32p, u32p, u32, u32p, u32p, u32p,
                                  u8p, u64, u32, u32, u64]
         lib.cm_dump.argtypes = [vp, u32p, u32p, u32p, u32p, ctypes.POINTER(u64),
@@ -53,6 +54,11 @@ class CpuModel:
         if getattr(self, "h", None):
             self.lib.cm_destroy(self.h)
             self.h = None
+
+    def set_hinted(self, on):
+        """Writes of later batches start at hints from a read-only walk made before the
+        batch runs (the GPU write passes' start hints)."""
+        self.lib.cm_set_hinted(self.h, 1 if on else 0)
 
     @property
     def error(self):

@@ -262,9 +262,10 @@ class CpuModelBackend:
     """The shared state machine run on the host (tests/cpu_model); no scans."""
     name = "cpu_model"
 
-    def __init__(self, nthreads=0, ntiles=32, seed=0):
+    def __init__(self, nthreads=0, ntiles=32, seed=0, hinted=False):
         from cpu_model.model import CpuModel
         self.m = CpuModel(1 << 21)
+        self.m.set_hinted(hinted)
         self.nthreads, self.ntiles, self.seed = nthreads, ntiles, seed
 
     def _run(self, kind, keys, lens, maxK, vals=None, vlens=None, maxV=1, find=False, kinds=None,

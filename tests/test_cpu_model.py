@@ -59,13 +59,17 @@ def test_duplicates_in_batch(nthreads):
                   {k: exp[k] for k in ("find_values", "find_lens", "items")})
 
 
+@pytest.mark.parametrize("hinted", [False, True], ids=["from_root", "start_hints"])
 @pytest.mark.parametrize("nthreads", [0, 8])
-def test_random_mixed_batches(nthreads):
+def test_random_mixed_batches(nthreads, hinted):
+    """Mixed batches from the root, and with every write begun at the start hint of a
+    read-only walk made before the batch (as the GPU write passes begin them): hints go
+    stale while the batch splits nodes and pushes keys down a layer."""
     wl = W.MixedWorkload(20000, key_len=3, seed=11)
     # shared first slices force layers and suffix push-downs
     wl.keys[:, 0] %= 64
     wl.hashes = W.key_hash64(wl.keys, wl.lens)
-    b = S.CpuModelBackend(nthreads=nthreads)
+    b = S.CpuModelBackend(nthreads=nthreads, hinted=hinted)
     o = S.OracleBackend()
     mk, mv = 3, 2
     assert np.array_equal(b.insert(wl.keys, wl.lens, wl.values, None, mk, mv),
@@ -77,4 +81,30 @@ def test_random_mixed_batches(nthreads):
         exp = o.mixed(bt["types"], bt["keys"], bt["lens"], bt["values"], vl, mk, mv)
         for g, e in zip(got, exp):
             assert np.array_equal(g, e)
+    assert b.items() == o.items()
+
+
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_hinted_growth_inserts(nthreads):
+    """Insert batches up to 4x the tree's size with start hints from the tree before the
+    batch (the GPU growth stages): the hinted writes move right past nodes that split after
+    the walk and find their parents from stale level-1 hints; the result equals the oracle's."""
+    rng = np.random.default_rng(5)
+    keys, lens = W.unique_random_keys(rng, 60000, 2, 6)
+    keys[:, 0] %= 512  # shared first slices: layers and push-downs under the hints
+    _, first = np.unique(keys, axis=0, return_index=True)
+    keys = keys[np.sort(first)]
+    lens = lens[np.sort(first)]
+    vals = rng.integers(0, 1 << 32, size=(len(lens), 2), dtype=np.uint64).astype(np.uint32)
+    b = S.CpuModelBackend(nthreads=nthreads, hinted=True)
+    o = S.OracleBackend()
+    mk, mv = keys.shape[1], 2
+    done, step = 0, 256
+    while done < len(lens):
+        sl = slice(done, min(done + step, len(lens)))
+        assert np.array_equal(b.insert(keys[sl], lens[sl], vals[sl], None, mk, mv),
+                              o.insert(keys[sl], lens[sl], vals[sl], None, mk, mv))
+        done = sl.stop
+        step *= 4
+    assert b.m.error == 0
     assert b.items() == o.items()
