@@ -359,12 +359,20 @@ class masstree_server {
 
   // Ring of scan result rows.  A wave's rows are contiguous (the tail of the ring is
   // skipped when they would wrap); allocations are released in allocation order, so the
-  // ring's free space is [head, tail + capacity).
+  // ring's free space is [head, tail + capacity).  The head only moves forward, so rows
+  // of completed requests are reused only when the ring wraps.  (Restarting at row 0
+  // whenever the ring was empty overwrote the rows of requests that had just completed.)
+  // An empty ring that must wrap for a wave starts it at row 0, so a wave as large as
+  // the ring always fits.
   bool reserve_ring(uint64_t rows, uint64_t& row0, uint64_t& id) {
     const uint64_t cap = cfg_.ring_rows;
-    if (ring_allocs_.empty()) ring_head_ = ring_tail_ = 0;  // empty ring: start at row 0
     const uint64_t start = ring_head_ % cap;
-    const uint64_t skip = start + rows > cap ? cap - start : 0;
+    uint64_t skip = start + rows > cap ? cap - start : 0;
+    if (skip && ring_allocs_.empty()) {  // every row is free: wrap now
+      ring_head_ += skip;
+      ring_tail_ = ring_head_;
+      skip = 0;
+    }
     if (ring_head_ + skip + rows - ring_tail_ > cap) return false;
     ring_head_ += skip;
     row0 = ring_head_ % cap;
