@@ -300,12 +300,15 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_find_kernel(tree_view tv
 // so one combined kernel executes the interior, border and record code paths on nearly
 // every step.  Here pass 1 (narrow_descend_kernel) walks interior and border nodes
 // through the trie layers and parks each lookup that ends at a record:
-// out_value_lengths[i] = kPendingRecord | record index, and park[i - lo] = keystate << 16
-// | layer, lo being the first request of the launch (of its range, in the batched form).
+// out_value_lengths[i] = kPendingRecord | record index, and park[i - lo] = keystate << 30
+// | layer << 15 | tail length (key length - layer - 1), lo being the first request of the
+// launch (of its range, in the batched form); so pass 2 reads no key lengths.  Keys of
+// 2^15 slices or more take keystate << 16 | layer, and pass 2 reads their lengths.
 // Pass 2 (narrow_record_kernel) reads every parked record in one uniform sweep.  Results
 // equal the one-pass kernel's; values of absent keys are not written by either pass.
 // In a mixed batch both passes answer the finds only.
 constexpr u32 kPendingRecord = 0x80000000u;  // record indices stay below 2^31
+constexpr u32 kParkPacked = 1u << 15;        // max_key_length below this: packed park words
 constexpr u32 kNoOutput = 0xFFFFFFFEu;       // a request the find passes do not answer
 constexpr u32 kHintDone = 0xFFFFFFFDu;       // a write request: its start hint is written
 
@@ -557,7 +560,9 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
           if (x.j == 0 && maxV >= 1) a.out_values[request(k) * (u64)maxV] = v;
           len = 1;
         } else if (v < nslabs) {
-          if (x.j == 0) park[idx[k] - lo] = (ks << 16) | l;
+          if (x.j == 0)
+            park[idx[k] - lo] = maxK < kParkPacked ? (ks << 30) | (l << 15) | (klen[k] - l - 1)
+                                                   : (ks << 16) | l;
           len = kPendingRecord | v;
         } else {
           node[k] = v;  // reported below
@@ -610,12 +615,13 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_record_kernel(tree_view 
   }
   if (lo >= hi) return;
   u64 i = lo + (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
+  const bool packed = maxK < kParkPacked;  // park words carry the tail length
   u32 next_w, next_pk, next_len, next_kind = op_find;
   auto fetch = [&](u64 e) {
     const u64 c = e < hi ? e : hi - 1;  // clamped: the loads are unconditional
     next_w = a.out_value_lengths[c];
     next_pk = park[c - lo];
-    next_len = a.key_lengths ? a.key_lengths[c] : maxK;
+    next_len = (!packed && a.key_lengths) ? a.key_lengths[c] : maxK;
     if (a.types) next_kind = a.types[c];
   };
   fetch(i);
@@ -630,13 +636,14 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_record_kernel(tree_view 
     // (in a mixed batch the length word of a write request is its input value length)
     if (!(w & kPendingRecord) || kd != op_find) continue;
     const u32 rec = w & ~kPendingRecord;  // below the pool size: checked by pass 1
-    const u32 rkind = pk >> 16, l = pk & 0xFFFFu;
+    const u32 rkind = packed ? pk >> 30 : pk >> 16;
+    const u32 l = packed ? (pk >> 15) & 0x7FFFu : pk & 0xFFFFu;
     const u32* key = a.keys + i * (u64)maxK;
     u32* out = a.out_values + i * (u64)maxV;
     node_piece<S> cur = load_piece<T>(pool, rec, x.j);
     // a suffix record matches only with a tail of exactly klen - l - 1 slices; the key's
     // tail words for the first slab's data words 1 .. 30 load beside the record
-    const u32 want = rkind == kSuffix ? klen - l - 1 : 0u;
+    const u32 want = rkind == kSuffix ? (packed ? pk & 0x7FFFu : klen - l - 1) : 0u;
     u32 kw[W];
 #pragma unroll
     for (u32 e = 0; e < W; e++) {
