@@ -49,6 +49,7 @@
 #include <vector>
 
 #include "gpu_masstree.hpp"
+#include "scan_ring.hpp"
 
 namespace indexingpu_amd {
 
@@ -204,9 +205,7 @@ class masstree_server {
     used_slabs_ = tree_.allocator_.num_allocated_slabs();
     reserved_ = 0;
     census_pending_ = false;
-    ring_head_ = ring_tail_ = 0;
-    ring_allocs_.clear();
-    ring_next_id_ = ring_front_id_ = 0;
+    ring_.reset(cfg_.ring_rows);
     census_due_ = census_fresh_ = aux_busy_ = false;
     draining_ = -1;
     waves_.clear();
@@ -237,7 +236,7 @@ class masstree_server {
         for (uint64_t r = w.first; r < w.last; r++) out[r].done_us = t;
         completed += w.last - w.first;
         w.done = true;
-        if (w.ring_rows) release_ring(w.ring_id);
+        if (w.ring_rows) ring_.release(w.ring_id);
         reserved_ -= w.reserved;
         used_slabs_ += w.reserved;  // pessimistic until the next census
         if (census_pending_) completed_in_census_ += w.reserved;
@@ -269,7 +268,7 @@ class masstree_server {
         uint64_t row0 = 0, ring_id = 0;
         const uint64_t rows = scans * (uint64_t)a.scan_max_count;
         if (cfg_.coalesce) {
-          if (last > admitted && rows && !reserve_ring(rows, row0, ring_id)) last = admitted;
+          if (last > admitted && rows && !ring_.reserve(rows, row0, ring_id)) last = admitted;
         } else {
           row0 = req[admitted].scan_offset * (uint64_t)a.scan_max_count;
         }
@@ -354,39 +353,6 @@ class masstree_server {
       stacks_.assign(streams_.size(), nullptr);
       for (auto& p : stacks_) IGPU_HIP_TRY(hipMalloc(&p, words * 4));
       stack_words_ = words;
-    }
-  }
-
-  // Ring of scan result rows.  A wave's rows are contiguous (the tail of the ring is
-  // skipped when they would wrap); allocations are released in allocation order, so the
-  // ring's free space is [head, tail + capacity).  The head only moves forward, so rows
-  // of completed requests are reused only when the ring wraps.  (Restarting at row 0
-  // whenever the ring was empty overwrote the rows of requests that had just completed.)
-  // An empty ring that must wrap for a wave starts it at row 0, so a wave as large as
-  // the ring always fits.
-  bool reserve_ring(uint64_t rows, uint64_t& row0, uint64_t& id) {
-    const uint64_t cap = cfg_.ring_rows;
-    const uint64_t start = ring_head_ % cap;
-    uint64_t skip = start + rows > cap ? cap - start : 0;
-    if (skip && ring_allocs_.empty()) {  // every row is free: wrap now
-      ring_head_ += skip;
-      ring_tail_ = ring_head_;
-      skip = 0;
-    }
-    if (ring_head_ + skip + rows - ring_tail_ > cap) return false;
-    ring_head_ += skip;
-    row0 = ring_head_ % cap;
-    ring_head_ += rows;
-    id = ring_next_id_++;
-    ring_allocs_.push_back({ring_head_, false});
-    return true;
-  }
-  void release_ring(uint64_t id) {
-    ring_allocs_[(size_t)(id - ring_front_id_)].freed = true;
-    while (!ring_allocs_.empty() && ring_allocs_.front().freed) {
-      ring_tail_ = ring_allocs_.front().end;
-      ring_allocs_.pop_front();
-      ring_front_id_++;
     }
   }
 
@@ -603,12 +569,7 @@ class masstree_server {
   bool census_pending_ = false, census_due_ = false, census_fresh_ = false;
   unsigned long long* census_dev_ = nullptr;
   unsigned long long* census_host_ = nullptr;
-  struct ring_alloc {
-    uint64_t end;
-    bool freed;
-  };
-  std::deque<ring_alloc> ring_allocs_;
-  uint64_t ring_head_ = 0, ring_tail_ = 0, ring_next_id_ = 0, ring_front_id_ = 0;
+  scan_ring ring_;  // rows of the scan result ring (scan_ring.hpp)
 };
 
 }  // namespace indexingpu_amd
