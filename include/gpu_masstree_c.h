@@ -223,8 +223,9 @@ typedef struct {
 /* Device arena of a trace: point ops in gmt_mixed_batch's layout, scans in gmt_scan's
  * (per-query limits <= scan_max_count), all requests back to back in arrival order.
  * Scan results go to the ring: request r's rows start at outcome[r].ring_row, each scan
- * taking scan_max_count rows.  With coalesce = 0 the ring must hold every scan of the
- * trace (rows are at scan index * scan_max_count). */
+ * taking scan_max_count rows.  A completed request's rows stay until the ring wraps past
+ * them (a ring that holds every scan of the trace keeps all of them).  With coalesce = 0
+ * the ring must hold every scan of the trace (rows are at scan index * scan_max_count). */
 typedef struct {
   const uint8_t* types;
   const uint32_t* keys;
