@@ -227,3 +227,34 @@ def test_mixed_finds_pass_long_keys_and_values(find_tile):
     for a, b in zip(g.mixed(types, bk, bl, bv, bvl, mk, mv), o.mixed(types, bk, bl, bv, bvl, mk, mv)):
         _eq(a, b)
     assert g.items() == o.items()
+
+
+def test_keys_of_2_15_slices_or_more():
+    """max_key_length >= 2^15: the two-pass find parks keystate and layer only and its
+    record pass reads the key lengths (shorter maximum lengths pack the tail length into the
+    park word instead).  Short and very long keys sharing first slices (suffix records with
+    tails of tens of thousands of slices), found present and absent, against the oracle."""
+    rng = np.random.default_rng(41)
+    n, mk, mv = 96, 40_000, 3
+    lens = np.concatenate([rng.integers(1, 7, size=32), rng.integers(32_768, mk + 1, size=64)])
+    lens = lens.astype(np.uint32)
+    keys = rng.integers(0, 1 << 32, size=(n, mk), dtype=np.uint64).astype(np.uint32)
+    keys[:, 0] %= 4  # shared first slices: suffix records, push-downs
+    keys[np.arange(mk)[None, :] >= lens[:, None]] = 0
+    h = W.key_hash64(keys, lens)
+    _, first = np.unique(h, return_index=True)
+    first = np.sort(first)
+    keys, lens = np.ascontiguousarray(keys[first]), lens[first]
+    n = len(lens)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    vlens = np.full(n, mv, np.uint32)
+    g, o = S.GpuBackend(16, pool_bytes=512 << 20), S.OracleBackend()
+    _eq(g.insert(keys, lens, vals, vlens, mk, mv), o.insert(keys, lens, vals, vlens, mk, mv))
+    # absent keys: the last slice of each key changed (same length, same tail length)
+    absent = keys.copy()
+    absent[np.arange(n), lens - 1] ^= 0x5A5A5A5A
+    for q in (keys, absent):
+        for a, b in zip(g.find(q, lens, mk, mv), o.find(q, lens, mk, mv)):
+            _eq(a, b)
+    ok, *_, msg = g.validate()
+    assert ok, msg
