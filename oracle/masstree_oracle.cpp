@@ -29,6 +29,15 @@
 //     in one batch, so any order gives the same outcome.
 // Lengths above 65535 slices are rejected by the GPU index (16-bit record
 // lengths, include/suffix_node_subwarp.hpp:706); the oracle accepts them.
+//
+// ORC_PLANT (compile-time, 0 = off) plants one known mistake, so the pinning tests
+// can show that they catch it: 1 = find reports the truncated value length,
+// 2 = scan reports the truncated key length, 3 = scan reports the truncated value
+// length, 4 = find copies one slice fewer than min(len, max_value_length).
+// The library the tests and the benchmark load is always built with 0.
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -57,7 +66,15 @@ inline Val make_val(const uint32_t* vals, uint32_t maxV, const uint32_t* lens, u
 inline void write_value(const Val& v, uint32_t* out_vals, uint32_t maxV, uint64_t slot) {
   if (!out_vals) return;
   uint32_t n = std::min<uint32_t>((uint32_t)v.size(), maxV);
+  if (ORC_PLANT == 4 && n > 0) n -= 1;
   std::memcpy(out_vals + slot * (uint64_t)maxV, v.data(), n * sizeof(uint32_t));
+}
+// Lengths are reported in full even when the copy was cut to the output row
+// (include/suffix_node_subwarp.hpp:519-544 returns the stored length after copying
+// min(len, max) slices; include/gpu_masstree.hpp:227-240 passes it through).
+inline uint32_t reported_length(std::size_t stored, uint32_t max_out, bool planted) {
+  if (planted) return (uint32_t)std::min<std::size_t>(stored, max_out);
+  return (uint32_t)stored;
 }
 }  // namespace
 
@@ -120,7 +137,7 @@ void orc_find(void* h, const uint32_t* keys, uint32_t maxK, const uint32_t* key_
     if (it == m.end()) {
       value_lengths[i] = 0;
     } else {
-      value_lengths[i] = (uint32_t)it->second.size();
+      value_lengths[i] = reported_length(it->second.size(), maxV, ORC_PLANT == 1);
       write_value(it->second, values, maxV, i);
     }
   }
@@ -143,9 +160,11 @@ void orc_scan(void* h, const uint32_t* lower_keys, const uint32_t* lower_lengths
     for (auto it = m.lower_bound(lo); it != m.end() && c < limit; ++it, ++c) {
       if (upper_keys && hi < it->first) break;
       uint64_t slot = q * (uint64_t)max_count + c;
-      if (value_lengths) value_lengths[slot] = (uint32_t)it->second.size();
+      if (value_lengths)
+        value_lengths[slot] = reported_length(it->second.size(), maxV, ORC_PLANT == 3);
       write_value(it->second, values, maxV, slot);
-      if (out_key_lengths) out_key_lengths[slot] = (uint32_t)it->first.size();
+      if (out_key_lengths)
+        out_key_lengths[slot] = reported_length(it->first.size(), maxK, ORC_PLANT == 2);
       if (out_keys) {
         uint32_t nk = std::min<uint32_t>((uint32_t)it->first.size(), maxK);
         std::memcpy(out_keys + slot * (uint64_t)maxK, it->first.data(), nk * sizeof(uint32_t));
@@ -178,7 +197,7 @@ void orc_mixed(void* h, const uint8_t* types, const uint32_t* keys, uint32_t max
         if (it == m.end()) {
           value_lengths[i] = 0;
         } else {
-          value_lengths[i] = (uint32_t)it->second.size();
+          value_lengths[i] = reported_length(it->second.size(), maxV, ORC_PLANT == 1);
           write_value(it->second, values, maxV, i);
         }
         break;

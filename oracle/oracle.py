@@ -13,7 +13,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
-_lib = None
+_libs = {}
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
 u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -24,13 +24,15 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
-def _load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(_LIB_PATH):
+def _load(path=None):
+    """Loads liboracle.so (or, for the pinning tests, a variant built with a planted
+    mistake at ``path``)."""
+    path = path or _LIB_PATH
+    if path in _libs:
+        return _libs[path]
+    if path == _LIB_PATH and not os.path.exists(path):
         build()
-    lib = ctypes.CDLL(_LIB_PATH)
+    lib = ctypes.CDLL(path)
     vp = ctypes.c_void_p
     u32 = ctypes.c_uint32
     u64 = ctypes.c_uint64
@@ -49,7 +51,7 @@ def _load():
     lib.orc_dump.argtypes = [vp, u32p, u32p, u32p, u32p, ctypes.POINTER(u64),
                              ctypes.POINTER(u64)]
     lib.orc_dump.restype = u64
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -70,8 +72,8 @@ def _p8(a):
 class Oracle:
     """std::map replay with the gpu_masstree batch signatures (numpy in/out)."""
 
-    def __init__(self):
-        self._lib = _load()
+    def __init__(self, lib_path=None):
+        self._lib = _load(lib_path)
         self._h = self._lib.orc_create()
 
     def __del__(self):

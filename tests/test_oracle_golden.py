@@ -6,6 +6,7 @@ Expectations are the closed forms asserted by test/unittest_masstree.cu:318-708
 golden vectors.
 """
 import itertools
+import os
 import random
 
 import numpy as np
@@ -219,3 +220,166 @@ def test_planted_scan_mistake_is_caught():
     srt = sorted(keys)
     exclusive = [next((k for k in srt if k > lo), None) for lo in keys]
     assert first != exclusive
+
+
+# ------------------------------------------------------- truncated output rows
+# The reference copies min(len, max) slices into an output row and reports the full
+# stored length (include/suffix_node_subwarp.hpp:519-544 for values,
+# include/gpu_masstree.hpp:227-240 and :250-395 for find and scan).  The model below
+# stores values longer than the rows find, scan and mixed batches write into, and keys
+# longer than a scan's output rows; every slot past min(len, max) must keep its fill.
+FILL = 0xA5A5A5A5
+SHORT = [k for k in UNIVERSE if len(k) <= 2]
+
+
+def _check_truncation_model(o, seed):
+    from oracle.oracle import Oracle  # noqa: F401  (o is an Oracle)
+    rnd = random.Random(seed)
+    model = {}
+    maxk, maxv_in = 3, 5
+    for _ in range(40):
+        op = rnd.choice(["insert", "upsert", "find", "find", "scan", "scan", "mixed", "erase"])
+        if op in ("insert", "upsert", "erase"):
+            keys = rnd.sample(UNIVERSE, rnd.randint(1, 24))
+            ka, kl = _pack(keys, maxk)
+            if op == "erase":
+                r = o.erase(ka.reshape(-1), maxk, kl, len(keys))
+                for i, k in enumerate(keys):
+                    assert r[i] == (k in model)
+                    model.pop(k, None)
+                continue
+            vals = [tuple(rnd.randrange(1 << 32) for _ in range(rnd.randint(0, maxv_in)))
+                    for _ in keys]
+            va = np.zeros((len(keys), maxv_in), np.uint32)
+            for i, v in enumerate(vals):
+                va[i, :len(v)] = v
+            vl = np.array([len(v) for v in vals], np.uint32)
+            r = o.insert(ka.reshape(-1), maxk, kl, va.reshape(-1), maxv_in, vl, len(keys),
+                         update_if_exists=(op == "upsert"))
+            for i, k in enumerate(keys):
+                assert r[i] == (op == "upsert" or k not in model)
+                if op == "upsert" or k not in model:
+                    model[k] = vals[i]
+        elif op == "find":
+            keys = rnd.sample(UNIVERSE, rnd.randint(1, 24))
+            ka, kl = _pack(keys, maxk)
+            mv = rnd.choice([1, 2, 3])
+            v, l = o.find(ka.reshape(-1), maxk, kl, mv, len(keys), fill=FILL)
+            v = v.reshape(len(keys), mv)
+            for i, k in enumerate(keys):
+                if k not in model:
+                    assert l[i] == 0 and (v[i] == FILL).all()
+                    continue
+                exp = model[k]
+                n = min(len(exp), mv)
+                assert l[i] == len(exp)  # the full stored length, not n
+                assert tuple(v[i, :n]) == exp[:n]
+                assert (v[i, n:] == FILL).all()
+        elif op == "scan":
+            # output key rows of 2 slices: stored keys of length 3 are cut to 2
+            mks = 2
+            lows = rnd.sample(SHORT, rnd.randint(1, 8))
+            la, ll = _pack(lows, mks)
+            lim, mv = rnd.randint(1, 6), rnd.choice([1, 2, 3])
+            c, v, vlen, ok, okl = o.scan(la.reshape(-1), ll, mks, lim, len(lows),
+                                         max_value_length=mv, fill=FILL)
+            srt = sorted(model)
+            for q, lo in enumerate(lows):
+                exp = [k for k in srt if k >= lo][:lim]
+                assert c[q] == len(exp)
+                for j in range(lim):
+                    s = q * lim + j
+                    if j >= len(exp):
+                        assert okl[s] == FILL and vlen[s] == FILL
+                        continue
+                    k = exp[j]
+                    nk, nv = min(len(k), mks), min(len(model[k]), mv)
+                    assert okl[s] == len(k) and vlen[s] == len(model[k])
+                    assert tuple(ok[s * mks:s * mks + nk]) == k[:nk]
+                    assert (ok[s * mks + nk:(s + 1) * mks] == FILL).all()
+                    assert tuple(v[s * mv:s * mv + nv]) == model[k][:nv]
+                    assert (v[s * mv + nv:(s + 1) * mv] == FILL).all()
+        else:
+            # a mixed batch with rows of 2 value slices: finds of longer stored values
+            # are cut, inserts over a stored zero-length value fail (the key exists)
+            mv = 2
+            keys = rnd.sample(UNIVERSE, rnd.randint(1, 24))
+            ka, kl = _pack(keys, maxk)
+            types = np.array([rnd.choice([0, 1, 2, 2, 3]) for _ in keys], np.uint8)
+            vals = [tuple(rnd.randrange(1 << 32) for _ in range(rnd.randint(0, mv)))
+                    for _ in keys]
+            va = np.full((len(keys), mv), FILL, np.uint32)
+            for i, vv in enumerate(vals):
+                va[i, :len(vv)] = vv
+            vl = np.array([len(vv) for vv in vals], np.uint32)
+            v, l, r = o.mixed(types, ka.reshape(-1), maxk, kl, va.reshape(-1), mv, vl, len(keys))
+            v = v.reshape(len(keys), mv)
+            for i, k in enumerate(keys):
+                t = types[i]
+                if t == 0:
+                    assert r[i] == (k not in model)
+                    model.setdefault(k, vals[i])
+                elif t == 1:
+                    assert r[i] == (k in model)
+                    model.pop(k, None)
+                elif t == 2:
+                    if k not in model:
+                        assert l[i] == 0
+                        continue
+                    exp = model[k]
+                    n = min(len(exp), mv)
+                    assert l[i] == len(exp)
+                    assert tuple(v[i, :n]) == exp[:n]
+                    # slots past n keep what the request carried in
+                    assert tuple(v[i, n:]) == tuple(va[i, n:])
+                else:
+                    assert r[i] == (k in model)
+                    if k in model:
+                        model[k] = vals[i]
+        assert o.items() == sorted(model.items())
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_truncation_against_python_model(seed):
+    from oracle.oracle import Oracle
+    _check_truncation_model(Oracle(), seed)
+
+
+def test_zero_length_value_blocks_insert():
+    """gpu_masstree.hpp:607-617 + :231: a zero-length value is stored (LONGVAL of length
+    0), reads back as length 0, and a later insert of the key fails in a plain insert
+    and in a mixed batch alike; erase then succeeds."""
+    from oracle.oracle import Oracle
+    o = Oracle()
+    ka, kl = _pack([(7, 8)], 2)
+    z = np.zeros(2, np.uint32)
+    assert o.insert(ka.reshape(-1), 2, kl, z, 2, np.zeros(1, np.uint32), 1)[0] == 1
+    v, l = o.find(ka.reshape(-1), 2, kl, 2, 1, fill=FILL)
+    assert l[0] == 0 and (v == FILL).all()
+    assert o.insert(ka.reshape(-1), 2, kl, z + 5, 2, np.ones(1, np.uint32), 1)[0] == 0
+    _, _, r = o.mixed(np.array([0], np.uint8), ka.reshape(-1), 2, kl, z + 5, 2,
+                      np.ones(1, np.uint32), 1)
+    assert r[0] == 0
+    assert o.items() == [((7, 8), ())]
+    assert o.erase(ka.reshape(-1), 2, kl, 1)[0] == 1 and o.items() == []
+
+
+@pytest.mark.parametrize("plant", [1, 2, 3, 4])
+def test_planted_truncation_mistakes_are_caught(plant, tmp_path):
+    """Builds the oracle with one planted mistake (ORC_PLANT: a truncated length
+    reported by find or by scan, one slice too few copied) and checks that the model
+    above rejects it."""
+    import subprocess
+    from oracle.oracle import Oracle
+    src = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                       "masstree_oracle.cpp")
+    lib = str(tmp_path / f"liboracle_plant{plant}.so")
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-fPIC", "-shared",
+                           f"-DORC_PLANT={plant}", "-o", lib, src])
+    caught = 0
+    for seed in range(6):
+        try:
+            _check_truncation_model(Oracle(lib_path=lib), seed)
+        except AssertionError:
+            caught += 1
+    assert caught > 0
