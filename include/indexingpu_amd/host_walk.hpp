@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <functional>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "layout.hpp"
@@ -22,6 +23,7 @@ struct walk_stats {
   uint64_t layers = 0;
   uint64_t record_slabs = 0;
   uint64_t max_depth = 0;       // deepest B-link level count in any layer
+  uint64_t unlinked = 0;        // non-root nodes with no parent slot (reached by sibling links)
   std::string error;            // empty when every invariant holds
 };
 
@@ -104,7 +106,9 @@ class host_walker {
       level--;
     }
     // check each interior level: bounds ascending, children at level-1
+    std::unordered_set<uint32_t> linked;  // children named by the level above
     for (size_t li = 0; li + 1 < level_heads.size(); li++) {
+      std::unordered_set<uint32_t> next_linked;
       uint32_t n = level_heads[li];
       uint32_t prev_high = 0;
       bool first = true;
@@ -120,7 +124,9 @@ class host_walker {
         uint32_t tag = 0;
         for (uint32_t i = 0; i < kTagSlots; i++) tag |= (uint32_t)(p[i] >> 63) << i;
         if (tag != hdr_tag(hh)) return fail("interior version tag mismatch");
+        if (li > 0 && !linked.count(n)) stats_.unlinked++;
         for (uint32_t i = 0; i < nk; i++) {
+          next_linked.insert(child_of(p[i]));
           uint32_t b = slot_lo(p[i]);
           if (i > 0 && b <= slot_lo(p[i - 1])) return fail("interior bounds not ascending");
           if (!first && i == 0 && b <= prev_high) return fail("interior bound below left high");
@@ -137,6 +143,7 @@ class host_walker {
           return fail("rightmost interior node without max bound");
         n = sib;
       }
+      linked.swap(next_linked);
     }
     // walk the border level left to right
     uint32_t n = level_heads.back();
@@ -157,6 +164,7 @@ class host_walker {
       if (nk > kBorderCap) return fail("bad border nkeys");
       uint32_t high = slot_lo(p[kHighSlot]);
       uint32_t ksb = slot_hi(p[kHighSlot]);
+      if (level_heads.size() > 1 && !linked.count(n)) stats_.unlinked++;
       stats_.nodes++;
       stats_.border_nodes++;
       for (uint32_t i = 0; i < nk; i++) {

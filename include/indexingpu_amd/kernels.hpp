@@ -68,6 +68,8 @@ struct batch_args {
   // none).  The write launch then starts each write at its border node instead of the
   // root.
   u32* hints;
+  // insert / upsert with enable_suffix = false (layers instead of suffix records)
+  u32 no_suffix;
 };
 constexpr u32 kHintWords = 4;
 
@@ -84,6 +86,7 @@ __device__ __forceinline__ void load_request(const batch_args& a, u64 i, request
   rq.value_length = a.value_lengths ? a.value_lengths[i] : a.max_value_length;
   rq.out_value = a.out_values ? a.out_values + i * (u64)a.max_value_length : nullptr;
   rq.max_value_length = a.out_values ? a.max_value_length : 0u;
+  rq.no_suffix = a.no_suffix;
 }
 
 // After a corruption or livelock guard fired, the remaining requests of the launch are
@@ -713,19 +716,8 @@ __global__ void drain_kernel(slab_pool_view pool, retire_view rv) {
   const u32 raw = *rv.count;
   const u32 count = raw < rv.capacity ? raw : rv.capacity;
   const u32 stride = gridDim.x * blockDim.x;
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-    const u32 r = rv.list[i];
-    if (r >= pool.num_slabs) continue;
-    const u32* w = reinterpret_cast<const u32*>(pool.pool + (u64)r * kSlots);
-    const u32 lw = w[0];
-    const u32 ns = rec_num_slabs(1 + rec_tail_len(lw) + rec_value_len(lw));
-    u32 cur = r;
-    for (u32 s = 0; s < ns && cur < pool.num_slabs; s++) {
-      const u32 nxt = reinterpret_cast<const u32*>(pool.pool + (u64)cur * kSlots)[kRecordNextWord];
-      dev::free_slab(pool, cur);
-      cur = nxt;
-    }
-  }
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+    dev::free_retired(pool, rv.list[i]);
 }
 
 __global__ void reset_kernel(retire_view rv) {

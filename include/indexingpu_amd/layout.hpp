@@ -118,6 +118,10 @@ IGPU_HD u32 rec_num_slabs(u32 total_words) {
   return (total_words + kRecordWordsPerSlab - 1) / kRecordWordsPerSlab;
 }
 
+// Retire-list entries: a record head index, or a node slab index with this bit set
+// (slab indices stay below 2^31).
+constexpr u32 kRetiredNode = 0x80000000u;
+
 IGPU_HD u64 slot_make(u32 lo, u32 hi) { return (u64)lo | ((u64)hi << 32); }
 IGPU_HD u32 slot_lo(u64 s) { return (u32)s; }
 IGPU_HD u32 slot_hi(u64 s) { return (u32)(s >> 32); }

@@ -4,18 +4,25 @@
 // simple_debra_reclaimer<buffer_size_per_block> and simple_dummy_reclaimer
 // (include/simple_debra_reclaim.hpp:32-110, include/simple_dummy_reclaim.hpp:26-64).
 //
-// Mechanism (this project's design).  The index never frees a node (erase does not
-// merge), so only records (key tails and long values) are ever retired.  A retired
-// record may still be read by an optimistic reader of the same launch, so it is not
-// freed in place: the retiring tile appends it to a device retire list, reserving
-// 16 entries at a time with one atomic.  After the launch, the host enqueues a drain
-// kernel on the same stream; stream order guarantees every reader of that launch has
-// finished, so the drain frees each retired record chain and resets the list.  This
-// is epoch-based reclamation with the launch as the epoch: no per-iteration announce
-// exchange, no block barriers inside the batch kernel.  A full list leaks the record
-// (counted in `leaked`) instead of blocking.  Launches that mutate one index must be
-// ordered on one stream, as the reference requires for its announce array
-// (include/simple_debra_reclaim.hpp:189,226).
+// Mechanism (this project's design).  A retired object (a record, or a node that an
+// erase unlinked) may still be read by an optimistic reader of the same launch, so it
+// is not freed in place.
+//   * Batch kernels: the retiring tile appends it to a device retire list, reserving 16
+//     entries at a time with one atomic.  After the launch, the host enqueues a drain
+//     kernel on the same stream; stream order guarantees every reader of that launch
+//     has finished, so the drain frees each retired object and resets the list.  This
+//     is epoch-based reclamation with the launch as the epoch: no per-iteration
+//     announce exchange, no block barriers inside the batch kernels.  A full list leaks
+//     the object (counted in `leaked`) instead of blocking.
+//   * User kernels of the device API that bracket their work with the reclaimer
+//     context's begin_critical_section / end_critical_section (device_context.hpp):
+//     DEBRA-style epochs inside the launch, so a long-running kernel reuses the slabs it
+//     retires.  Each block announces the global epoch it observed; a block whose
+//     announced epoch is current everywhere advances it; objects a block retired while
+//     announcing epoch e are freed by that block once it observes epoch e + 3 (three
+//     limbo bags per block).  drain_all hands what is left to the launch list.
+// Launches that mutate one index must be ordered on one stream, as the reference
+// requires for its announce array (include/simple_debra_reclaim.hpp:189,226).
 #pragma once
 #include <cstdint>
 
@@ -30,7 +37,21 @@ struct retire_view {
   u32* leaked;    // records dropped because the list was full
   u32 capacity;
   u32 enabled;    // 0: retire is a no-op (simple_dummy_reclaimer)
+  // In-kernel epoch reclamation for user kernels of the device API (the context's
+  // begin_critical_section / end_critical_section / drain_all; device_context.hpp).
+  u32* epoch;     // global epoch
+  u32* announce;  // per block of the launch: epoch << 1 | 1 while in a critical section
+  u32* bags;      // limbo bags, split evenly between the blocks of a launch
+  u32 bag_words;  // words of `bags`
+  u32 max_blocks; // entries of `announce`
+  // Set by a block's reclaimer context for its own tiles (null in the batch kernels):
+  // this block's region of `bags` (header of kBagHeader words, then 3 bags of bag_cap
+  // entries); retired objects go to the current bag instead of the launch list.
+  u32* bag;
+  u32 bag_cap;
 };
+constexpr u32 kBagHeader = 8;  // counts[3], epoch tags[3], -, current bag
+constexpr u32 kBagCurrent = 7;
 
 template <uint32_t buffer_size_per_block = 131072>
 class simple_debra_reclaimer {
@@ -38,6 +59,8 @@ class simple_debra_reclaimer {
   using size_type = uint32_t;
   using device_instance_type = retire_view;
   static constexpr uint32_t block_size_ = 256;
+  static constexpr uint32_t kMaxBlocks = 1u << 16;
+  static constexpr uint32_t kBagBlocks = 256;
 
   simple_debra_reclaimer() {
     int dev = 0, cus = 0;
@@ -52,6 +75,8 @@ class simple_debra_reclaimer {
   ~simple_debra_reclaimer() {
     if (view_.list) (void)hipFree(view_.list);
     if (view_.count) (void)hipFree(view_.count);
+    if (view_.announce) (void)hipFree(view_.announce);
+    if (view_.bags) (void)hipFree(view_.bags);
   }
   simple_debra_reclaimer(const simple_debra_reclaimer&) = delete;
   simple_debra_reclaimer& operator=(const simple_debra_reclaimer&) = delete;
@@ -72,6 +97,15 @@ class simple_debra_reclaimer {
     IGPU_HIP_TRY(hipMalloc(&view_.count, 256));
     IGPU_HIP_TRY(hipMemset(view_.count, 0, 256));
     view_.leaked = view_.count + 1;
+    view_.epoch = view_.count + 32;  // its own 128-byte line
+    // device-API epochs: announce entries for up to kMaxBlocks blocks per launch and
+    // buffer_size_per_block bytes of limbo bags for each of up to kBagBlocks blocks
+    view_.max_blocks = kMaxBlocks;
+    IGPU_HIP_TRY(hipMalloc(&view_.announce, 4ull * kMaxBlocks));
+    IGPU_HIP_TRY(hipMemset(view_.announce, 0, 4ull * kMaxBlocks));
+    view_.bag_words = (u32)((uint64_t)buffer_size_per_block / 4 * kBagBlocks);
+    IGPU_HIP_TRY(hipMalloc(&view_.bags, 4ull * view_.bag_words));
+    IGPU_HIP_TRY(hipMemset(view_.bags, 0, 4ull * view_.bag_words));
     IGPU_HIP_TRY(hipDeviceSynchronize());
   }
   retire_view view_{};

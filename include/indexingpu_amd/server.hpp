@@ -345,8 +345,9 @@ class masstree_server {
 
   void prepare_stacks(const serve_arena& a) {
     stack_layers_ = std::max<uint32_t>(a.max_key_length, 64u) + 1;
-    scan_blocks_ = resident_blocks(scan_kernel<16, true>);
-    const uint64_t words = (uint64_t)scan_blocks_ * (kBlockThreads / 16) * stack_layers_ * 3u;
+    scan_blocks_ = resident_blocks(scan_kernel<Tree::tile_size_, true>);
+    const uint64_t words =
+        (uint64_t)scan_blocks_ * (kBlockThreads / Tree::tile_size_) * stack_layers_ * 3u;
     if (stacks_.empty() || words > stack_words_) {
       IGPU_HIP_TRY(hipDeviceSynchronize());
       for (auto p : stacks_) (void)hipFree(p);
@@ -443,11 +444,11 @@ class masstree_server {
       sa.limits = a.scan_limits ? a.scan_limits + s0 : nullptr;
       sa.stack = stacks_[s];
       sa.stack_layers = stack_layers_;
-      const uint32_t tiles = kBlockThreads / 16;
+      const uint32_t tiles = kBlockThreads / Tree::tile_size_;
       const uint32_t blocks =
           (uint32_t)std::min<uint64_t>((sa.num_queries + tiles - 1) / tiles, scan_blocks_);
-      hipLaunchKernelGGL((scan_kernel<16, true>), dim3(blocks), dim3(kBlockThreads), 0, stream,
-                         v, sa);
+      hipLaunchKernelGGL((scan_kernel<Tree::tile_size_, true>), dim3(blocks), dim3(kBlockThreads),
+                         0, stream, v, sa);
     }
     IGPU_HIP_TRY(hipEventRecord(events_[s], stream));
     (void)st;

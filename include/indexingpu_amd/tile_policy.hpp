@@ -8,13 +8,24 @@
 // inserts, removals and splits are lane-parallel shuffles (ds_bpermute) followed by
 // one coalesced store of the whole node.
 //
-// Memory ordering (concurrent batches): node slots are read with agent-scope relaxed
-// atomic loads (one 128-byte line, one instruction), the header word is re-read after
-// an agent-scope acquire fence and must match the unlocked header seen in the line.
-// Writers lock the header with an acquire fetch_or, write the node with relaxed
-// stores, and release the lock with a release fetch_add that also advances the
-// version.  Read-only batches (find<concurrent=false>, scan) use plain cached loads,
-// as the reference's weak memory order does (include/gpu_masstree.hpp:206).
+// Memory ordering (concurrent batches).  Every access to shared tree memory is a
+// relaxed agent-scope atomic (sc1 loads and stores: served coherently across the eight
+// XCDs), and ordering between them comes from completion: dev::order() waits for every
+// outstanding memory operation of the wave (s_waitcnt vmcnt(0)) behind a compiler
+// barrier.  No C++ acquire/release fences are used on these paths: on gfx950 they are
+// whole-L2 write-back/invalidate operations.
+//   * Validated read: the whole line (header included) in one relaxed atomic load; a
+//     locked header fails; order(); the header is loaded again and must be unchanged.
+//   * Lock: compare-and-swap of the unlocked header seen by a validated read (relaxed),
+//     then order() before any store to the node.
+//   * Write + unlock: relaxed atomic stores of the node, order() (the stores are
+//     complete), then a relaxed fetch_add that clears the lock bit and advances the
+//     version (unlock_bump), or a fetch_and that only clears it (unlock_same).
+//   * Interior nodes may come from a plain cached load when unlocked and tagged with
+//     the header's version in both halves of the line (layout.hpp).
+// Read-only batches (find<concurrent=false>, scan) use plain cached loads, as the
+// reference's weak memory order does (include/gpu_masstree.hpp:206); kernel
+// boundaries order them against writes.
 #pragma once
 #include "hip_util.hpp"
 #include "layout.hpp"
@@ -210,6 +221,18 @@ struct wave_tile {
     const u32 m = ballot(slot < nk && q <= slot_lo(n.s));
     const u32 child = bcast(slot_hi(n.s), m ? (u32)__builtin_ctz(m) : 0u) & kChildMask;
     return m ? child : kInvalid;
+  }
+  // value / object index of border entry pos
+  __device__ __forceinline__ u32 entry_value(const node_t& n, u32 pos) const {
+    return bcast(slot_hi(n.s), pos);
+  }
+  // slot j of the first bound >= q and its child; false when q is above every bound
+  __device__ __forceinline__ bool find_cover(const node_t& n, u32 q, u32& j, u32& child) const {
+    const u32 nk = nkeys_of(n);
+    const u32 m = ballot(slot < nk && q <= slot_lo(n.s));
+    j = m ? (u32)__builtin_ctz(m) : 0u;
+    child = bcast(slot_hi(n.s), j) & kChildMask;
+    return m != 0;
   }
   __device__ __forceinline__ bool find_child_slot(const node_t& n, u32 child, u32& j) const {
     const u32 nk = nkeys_of(n);
@@ -417,6 +440,18 @@ struct wave_tile {
   }
   __device__ __forceinline__ void retire(u32 idx) {
     if (!rv.enabled) return;
+    if (rv.bag) {  // device API inside a critical section: this block's current bag
+      u32 ok = 0;
+      if (lane == 0) {
+        const u32 cur = rv.bag[kBagCurrent];
+        const u32 k = dev::atomic_add(&rv.bag[cur], 1u);
+        if (k < rv.bag_cap) {
+          rv.bag[kBagHeader + cur * rv.bag_cap + k] = idx;
+          ok = 1;
+        }
+      }
+      if (bcast(ok, 0)) return;  // a full bag falls back to the launch list
+    }
     if (rc.base == kInvalid || rc.used == kRetireChunk) {
       u32 b = 0;
       if (lane == 0) b = dev::atomic_add(rv.count, kRetireChunk);
@@ -485,6 +520,19 @@ struct wave_tile {
     }
     return vl;
   }
+  // Frees the slabs of a record chain whose allocation failed part way: the slabs from
+  // `head` up to (not including) `cur` are written and linked, `cur` is allocated but
+  // not written.  Nothing else has seen them.
+  __device__ __forceinline__ void rec_free_partial(u32 head, u32 cur) {
+    dev::order();  // this tile's stores of the chain are complete before it is read back
+    u32 idx = head;
+    while (idx != cur) {
+      const u32 nxt = bcast(slot_hi(dev::coherent_load(slab(idx) + slot)), kMetaSlot);
+      if (lane == 0) dev::free_slab(pool, idx);
+      idx = nxt;
+    }
+    if (lane == 0) dev::free_slab(pool, cur);
+  }
   // writes a new record whose data words come from word(d) (lane-parallel)
   template <class WordFn>
   __device__ __forceinline__ u32 rec_write(u32 total, WordFn word) {
@@ -496,7 +544,10 @@ struct wave_tile {
       u32 nxt = kInvalid;
       if (s + 1 < nslabs) {
         nxt = alloc_slab();
-        if (nxt == kInvalid) return kInvalid;
+        if (nxt == kInvalid) {
+          rec_free_partial(head, cur);
+          return kInvalid;
+        }
       }
       const u32 d0 = s * kRecordWordsPerSlab + 2 * slot, d1 = d0 + 1;
       const u32 v0 = word(d0, s, 0);
@@ -540,7 +591,10 @@ struct wave_tile {
       u32 nxt = kInvalid;
       if (s + 1 < nslabs) {
         nxt = alloc_slab();
-        if (nxt == kInvalid) return kInvalid;
+        if (nxt == kInvalid) {
+          rec_free_partial(head, cur);
+          return kInvalid;
+        }
       }
       // new word d' = 31 s + w takes old word d' + 1
       const u64 right = slot_from(oa, slot < 15 ? slot + 1 : 15);

@@ -64,6 +64,9 @@ struct request {
   u32 value_length;
   u32* out_value;       // find output (max_value_length slices)
   u32 max_value_length;
+  // insert / upsert with enable_suffix = false: a new key's slices past the current
+  // layer become one single-entry layer each instead of a suffix record
+  u32 no_suffix;
 };
 
 template <class NodeImage>
@@ -179,6 +182,43 @@ IGPU_HD bool make_value_entry(C& c, const request& rq, u32& ks, u32& val) {
   return val != kInvalid;
 }
 
+// enable_suffix = false (the reference's insert<..., enable_suffix = false>,
+// include/gpu_masstree.hpp:591-603): the key's slices key[from..len) become a chain of
+// new single-entry layer roots, one per slice, the last holding the value; returns the
+// first root (kInvalid when out of slabs: what was built is retired, nothing is
+// published).  The chain is private until the caller links it.
+template <class C>
+IGPU_HD u32 make_layer_chain(C& c, const request& rq, u32 from) {
+  using node_t = typename C::node_t;
+  u32 ks, val;
+  if (!make_value_entry(c, rq, ks, val)) return kInvalid;
+  for (u32 l = rq.key_length - 1;; l--) {
+    const u32 x = c.alloc_slab();
+    if (x == kInvalid) {
+      // retire the part built so far (layers l+1.. and the value record)
+      c.fence_release();
+      while (ks == kLink) {
+        node_t m;
+        c.read_node_locked(val, m);
+        const u32 nks = ks_get(c.ksbits(m), 0), nval = c.entry_value(m, 0);
+        c.retire(val | kRetiredNode);
+        ks = nks;
+        val = nval;
+      }
+      if (ks == kLongVal) c.retire(val);
+      return kInvalid;
+    }
+    node_t n;
+    c.make_single_root(n, c.key_slice(rq, l), ks, val);
+    c.write_node(x, n);
+    ks = kLink;
+    val = x;
+    if (l == from) break;
+  }
+  c.fence_release();
+  return val;
+}
+
 // Returns true when the step made progress (false: retry later).
 template <bool Validate, class C, class S>
 IGPU_HD bool op_step(C& c, const request& rq, S& st) {
@@ -193,7 +233,14 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
     if (st.t0 == 0) st.t0 = now;
     if (now - st.t0 > kMaxOpTime || st.steps > kMaxStepsPerOp) {
       c.set_error(kErrStepLimit);
-      op_finish(st, 0);
+      if (op_holds_locks(st)) {
+        // a split waiting for its parent holds both halves (written, not yet
+        // published); publish them: the right half is reachable through the left
+        // one's sibling link, and a later split of it links it from its left neighbour
+        c.unlock_bump(st.a0);
+        c.unlock_bump(st.a1);
+      }
+      op_finish(st, op_holds_locks(st) ? st.result : 0);
       return true;
     }
   }
@@ -320,7 +367,11 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         // insert a new entry
         u32 nks, nval;
         bool ok;
-        if (more) {
+        if (more && rq.no_suffix) {
+          nks = kLink;
+          nval = make_layer_chain(c, rq, st.layer + 1);
+          ok = nval != kInvalid;
+        } else if (more) {
           nks = kSuffix;
           nval = c.rec_create(rq.key, st.layer + 1, rq.key_length - st.layer - 1, rq.value,
                               rq.value_length);
@@ -524,17 +575,28 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       const u32 h = st.snap_hdr | kLockBit;
       c.set_header(n, h);
       u32 j;
+      u32 lc = st.a0;  // child left of the new separator
       if (!c.find_child_slot(n, st.a0, j)) {
-        // the separator's slot moved right with a split of this parent (or the parent
-        // is still being linked): continue with the right sibling, else from the root
-        const u32 sib = c.sibling(n);
-        c.unlock_same(st.node);
-        st.node = (sib != kInvalid) ? sib : st.layer_root;
-        st.phase = ph_parent_descend;
-        return true;
+        if (!c.find_cover(n, st.a2, j, lc)) {
+          // the separator lies right of this node (the parent split meanwhile, or is
+          // still being linked): continue with the right sibling, else from the root
+          const u32 sib = c.sibling(n);
+          c.unlock_same(st.node);
+          st.node = (sib != kInvalid) ? sib : st.layer_root;
+          st.phase = ph_parent_descend;
+          return true;
+        }
+        // This node covers the separator but has no slot for a0.  While a0 is locked
+        // by this operation no other split can be linking it, so a0 is a node whose
+        // own parent entry was never made (an earlier split ran out of slabs for its
+        // parent, or was abandoned by the time guard).  The slot covering the
+        // separator points at a node left of a0 on the same level, from which B-link
+        // move-right reaches a0.  Linking the new right half after that slot keeps
+        // every route valid: keys up to the separator still go through the left node
+        // and move right to a0; keys above it go straight to a1.
       }
       if (hdr_nkeys(h) < kInteriorCap) {
-        c.interior_insert(n, j, st.a2, st.a0, st.a1);
+        c.interior_insert(n, j, st.a2, lc, st.a1);
         c.write_node(st.node, n);
         c.unlock_bump(st.node);
         c.unlock_bump(st.a0);
@@ -547,15 +609,16 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         const u32 li = c.alloc_slab();
         const u32 ri = c.alloc_slab();
         if (li == kInvalid || ri == kInvalid) {
-          // out of slabs: the separator cannot be placed; the split halves stay
-          // reachable through the sibling link, so the tree is still correct
+          // out of slabs: the separator cannot be placed.  The right half stays
+          // reachable through the sibling link, so every route is still correct, and a
+          // later split of it links it from its left neighbour's slot (above).
           c.unlock_same(st.node);
           c.unlock_bump(st.a0);
           c.unlock_bump(st.a1);
           op_finish(st, st.result);
           return true;
         }
-        const u32 sep2 = c.interior_split(n, j, st.a2, st.a0, st.a1, left, right);
+        const u32 sep2 = c.interior_split(n, j, st.a2, lc, st.a1, left, right);
         c.set_header(left, hdr_make(c.nkeys_of(left), false, false, st.a3));
         c.set_sibling(left, ri);
         c.set_header(right, hdr_make(c.nkeys_of(right), false, false, st.a3));
@@ -579,7 +642,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         op_finish(st, st.result);
         return true;
       }
-      const u32 sep2 = c.interior_split(n, j, st.a2, st.a0, st.a1, left, right);
+      const u32 sep2 = c.interior_split(n, j, st.a2, lc, st.a1, left, right);
       c.set_header(right, hdr_make(c.nkeys_of(right), false, false, st.a3) | kLockBit);
       c.set_sibling(right, c.sibling(n));
       c.set_header(left, hdr_with_nkeys(h, c.nkeys_of(left)));

@@ -24,6 +24,7 @@ struct model {
   shared_pool pool;
   u32 root;
   u32 hinted = 0;  // writes start at hints from a read-only walk (as the GPU write passes)
+  u32 no_suffix = 0;  // inserts build layers instead of suffix records (enable_suffix=false)
   explicit model(u32 n) : pool(n) {
     cpu_tile t(pool);
     root = t.alloc_slab();
@@ -47,6 +48,7 @@ struct batch_args {
   uint32_t* out_values;
   uint32_t* out_vlens;
   uint8_t* results;
+  uint32_t no_suffix;
 };
 
 request make_request(const batch_args& a, uint64_t i, uint32_t* scratch_out) {
@@ -56,6 +58,7 @@ request make_request(const batch_args& a, uint64_t i, uint32_t* scratch_out) {
   rq.key = a.keys + i * (uint64_t)a.maxK;
   rq.value = a.values ? a.values + i * (uint64_t)a.maxV : nullptr;
   rq.value_length = a.vlens ? a.vlens[i] : a.maxV;
+  rq.no_suffix = a.no_suffix;
   rq.out_value = a.out_values ? a.out_values + i * (uint64_t)a.maxV : scratch_out;
   rq.max_value_length = a.out_values ? a.maxV : 0;
   // a find in a mixed batch writes its value over the request's value slots
@@ -155,6 +158,23 @@ uint32_t cm_error(void* h) { return static_cast<model*>(h)->pool.error.load(); }
 uint32_t cm_slabs_used(void* h) { return static_cast<model*>(h)->pool.next.load(); }
 uint64_t cm_retired(void* h) { return static_cast<model*>(h)->pool.retired.size(); }
 void cm_set_hinted(void* h, uint32_t on) { static_cast<model*>(h)->hinted = on; }
+void cm_set_no_suffix(void* h, uint32_t on) { static_cast<model*>(h)->no_suffix = on; }
+// Allocation failure injection (0 = off): every k-th allocation, and every k-th
+// allocation of a split inserting into its parent.  Also clears the error word.
+void cm_set_fail(void* h, uint32_t any_every, uint32_t parent_every) {
+  model& M = *static_cast<model*>(h);
+  M.pool.fail_any_every = any_every;
+  M.pool.fail_parent_every = parent_every;
+  M.pool.error.store(0);
+}
+// Non-root nodes that no parent slot names (reachable through sibling links only).
+uint64_t cm_unlinked(void* h) {
+  model& M = *static_cast<model*>(h);
+  host_walker w([&](uint32_t idx) -> const uint64_t* {
+    return idx < M.pool.num_slabs ? M.pool.slab(idx) : nullptr;
+  }, nullptr);
+  return w.run(M.root).unlinked;
+}
 
 // Runs one batch.  nthreads == 0: interleaved simulation with ntiles tiles and the
 // given scheduler seed; nthreads > 0: that many host threads (static partition).
@@ -165,7 +185,7 @@ void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32
               uint64_t seed) {
   model& M = *static_cast<model*>(h);
   batch_args a{kinds, default_kind, keys, maxK, klens, values, maxV, vlens,
-               out_values, out_vlens, results};
+               out_values, out_vlens, results, M.no_suffix};
   const std::vector<start_hint> hs = batch_hints(M, a, n);
   if (nthreads == 0) {
     std::mt19937_64 rng(seed);
@@ -188,7 +208,10 @@ void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32
     while (!active.empty()) {
       size_t pick = rng() % active.size();
       uint32_t t = active[pick];
-      if (st[t].phase != ph_done) op_step<true>(ctx[t], rq[t], st[t]);
+      if (st[t].phase != ph_done) {
+        ctx[t].in_parent = st[t].phase == ph_parent_lock;
+        op_step<true>(ctx[t], rq[t], st[t]);
+      }
       if (st[t].phase == ph_done) {
         store_result(a, rq[t], cur[t], st[t].result);
         cur[t] += ntiles;
@@ -213,6 +236,7 @@ void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32
         request r = make_request(a, i, &scratch);
         begin_request(M, c, s, r, hs, i);
         while (s.phase != ph_done) {
+          c.in_parent = s.phase == ph_parent_lock;
           if (!op_step<true>(c, r, s)) std::this_thread::yield();
         }
         store_result(a, r, i, s.result);

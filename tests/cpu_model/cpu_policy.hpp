@@ -27,6 +27,10 @@ struct shared_pool {
   u32 num_slabs = 0;
   std::atomic<u32> next{0};      // bump allocation (the model never reuses slabs)
   std::atomic<u32> error{0};
+  // injected allocation failures (0 = off): every k-th allocation fails, or every k-th
+  // allocation made while a split is inserting into its parent
+  u32 fail_any_every = 0, fail_parent_every = 0;
+  std::atomic<u32> any_allocs{0}, parent_allocs{0};
   std::mutex retire_mu;
   std::vector<u32> retired;
   explicit shared_pool(u32 n) : slots((size_t)n * kSlots, 0), num_slabs(n) {}
@@ -141,6 +145,17 @@ struct cpu_tile {
     for (u32 i = 0; i < nk; i++)
       if (q <= slot_lo(n.s[i])) return child_of(n.s[i]);
     return kInvalid;
+  }
+  u32 entry_value(const node_t& n, u32 pos) const { return slot_hi(n.s[pos]); }
+  bool find_cover(const node_t& n, u32 q, u32& j, u32& child) const {
+    u32 nk = nkeys_of(n);
+    for (u32 i = 0; i < nk; i++)
+      if (q <= slot_lo(n.s[i])) {
+        j = i;
+        child = child_of(n.s[i]);
+        return true;
+      }
+    return false;
   }
   bool find_child_slot(const node_t& n, u32 child, u32& j) const {
     u32 nk = nkeys_of(n);
@@ -285,7 +300,14 @@ struct cpu_tile {
   }
 
   // ---------------- slabs ----------------
+  bool in_parent = false;  // set by the driver while the operation is in ph_parent_lock
   u32 alloc_slab() {
+    if ((P.fail_any_every && P.any_allocs.fetch_add(1) % P.fail_any_every == 0) ||
+        (P.fail_parent_every && in_parent &&
+         P.parent_allocs.fetch_add(1) % P.fail_parent_every == 0)) {
+      set_error(kErrOutOfMemory);
+      return kInvalid;
+    }
     u32 i = P.next.fetch_add(1);
     if (i >= P.num_slabs) {
       set_error(kErrOutOfMemory);

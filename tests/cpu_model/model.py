@@ -28,6 +28,10 @@ def _load():
         lib.cm_retired.argtypes = [vp]
         lib.cm_retired.restype = u64
         lib.cm_set_hinted.argtypes = [vp, u32]
+        lib.cm_set_fail.argtypes = [vp, u32, u32]
+        lib.cm_set_no_suffix.argtypes = [vp, u32]
+        lib.cm_unlinked.argtypes = [vp]
+        lib.cm_unlinked.restype = u64
         lib.cm_batch.argtypes = [vp, u8p, u32, u32p, u32, u32p, u32p, u32, u32p, u32p, u32p,
                                  u8p, u64, u32, u32, u64]
         lib.cm_dump.argtypes = [vp, u32p, u32p, u32p, u32p, ctypes.POINTER(u64),
@@ -59,6 +63,21 @@ class CpuModel:
         """Writes of later batches start at hints from a read-only walk made before the
         batch runs (the GPU write passes' start hints)."""
         self.lib.cm_set_hinted(self.h, 1 if on else 0)
+
+    def set_no_suffix(self, on):
+        """Inserts of later batches build one layer per remaining key slice instead of a
+        suffix record (insert<..., enable_suffix = false>)."""
+        self.lib.cm_set_no_suffix(self.h, 1 if on else 0)
+
+    def set_fail(self, any_every=0, parent_every=0):
+        """Injected allocation failures: every k-th allocation fails, and every k-th
+        allocation of a split inserting into its parent (0 = off).  Clears the error."""
+        self.lib.cm_set_fail(self.h, any_every, parent_every)
+
+    @property
+    def unlinked(self):
+        """Non-root nodes that no parent slot names (reached through sibling links)."""
+        return int(self.lib.cm_unlinked(self.h))
 
     @property
     def error(self):

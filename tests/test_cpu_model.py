@@ -108,3 +108,110 @@ def test_hinted_growth_inserts(nthreads):
         step *= 4
     assert b.m.error == 0
     assert b.items() == o.items()
+
+
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_split_parent_out_of_memory(nthreads):
+    """Every allocation of a split inserting into its parent fails (a parent that must
+    split cannot): the new right halves keep no parent slot and are reached through
+    sibling links only.  Later splits of such nodes link them from the slot of their left
+    neighbour instead of searching for a slot that does not exist (which never ends),
+    inserts keep succeeding, and once slabs are available again every new right half is
+    linked.  Results equal the oracle's throughout."""
+    rng = np.random.default_rng(21)
+    keys, lens = W.unique_random_keys(rng, 40000, 1, 3)
+    vals = rng.integers(0, 1 << 32, size=(len(lens), 2), dtype=np.uint64).astype(np.uint32)
+    b = S.CpuModelBackend(nthreads=nthreads, allow_error=True)
+    o = S.OracleBackend()
+    mk, mv = keys.shape[1], 2
+    b.m.set_fail(parent_every=1)
+    half = len(lens) // 2
+    for lo in range(0, half, 2000):
+        sl = slice(lo, min(lo + 2000, half))
+        assert np.array_equal(b.insert(keys[sl], lens[sl], vals[sl], None, mk, mv),
+                              o.insert(keys[sl], lens[sl], vals[sl], None, mk, mv))
+    assert b.m.error == 1  # kErrOutOfMemory
+    orphans = b.m.unlinked
+    assert orphans > 100  # the root could not split: long sibling chains
+    assert b.items() == o.items()
+    fk = keys[:half]
+    got = b.find(fk, lens[:half], mk, mv)
+    exp = o.find(fk, lens[:half], mk, mv)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+    b.m.set_fail()
+    for lo in range(half, len(lens), 2000):
+        sl = slice(lo, min(lo + 2000, len(lens)))
+        assert np.array_equal(b.insert(keys[sl], lens[sl], vals[sl], None, mk, mv),
+                              o.insert(keys[sl], lens[sl], vals[sl], None, mk, mv))
+    assert b.m.error == 0
+    # new right halves, including those of unlinked nodes, all got parent slots
+    assert b.m.unlinked <= orphans
+    assert b.items() == o.items()
+
+
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_random_out_of_memory(nthreads):
+    """One allocation in seven fails, wherever it is made (records, border splits, layer
+    push-downs, parent splits): every insert either succeeds or reports false and leaves
+    no trace; the tree stays valid and equals the oracle fed only the successful inserts."""
+    rng = np.random.default_rng(22)
+    keys, lens = W.unique_random_keys(rng, 20000, 1, 5)
+    keys[:, 0] %= 256  # shared first slices: layers, suffix records and push-downs
+    _, first = np.unique(keys, axis=0, return_index=True)
+    keys, lens = keys[np.sort(first)], lens[np.sort(first)]
+    vl = rng.integers(1, 4, size=len(lens)).astype(np.uint32)
+    vals = rng.integers(0, 1 << 32, size=(len(lens), 3), dtype=np.uint64).astype(np.uint32)
+    b = S.CpuModelBackend(nthreads=nthreads, allow_error=True)
+    o = S.OracleBackend()
+    mk, mv = keys.shape[1], 3
+    b.m.set_fail(any_every=7)
+    ok_all = np.zeros(len(lens), bool)
+    for lo in range(0, len(lens), 2500):
+        sl = slice(lo, min(lo + 2500, len(lens)))
+        ok = b.insert(keys[sl], lens[sl], vals[sl], vl[sl], mk, mv).astype(bool)
+        ok_all[sl] = ok
+        idx = np.nonzero(ok)[0] + lo
+        o.insert(keys[idx], lens[idx], vals[idx], vl[idx], mk, mv)
+    assert 0 < ok_all.sum() < len(lens)
+    assert b.items() == o.items()
+    b.m.set_fail()
+    # the failed ones go in now
+    idx = np.nonzero(~ok_all)[0]
+    assert np.array_equal(b.insert(keys[idx], lens[idx], vals[idx], vl[idx], mk, mv),
+                          o.insert(keys[idx], lens[idx], vals[idx], vl[idx], mk, mv))
+    assert b.m.error == 0
+    assert b.items() == o.items()
+
+
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_inserts_without_suffix_records(nthreads):
+    """insert<..., enable_suffix = false>: a new key's remaining slices become one layer
+    each.  Finds, erases, updates and later suffix-record inserts over such layers give
+    the oracle's results, with writes begun from the root and from start hints."""
+    rng = np.random.default_rng(31)
+    keys, lens = W.unique_random_keys(rng, 12000, 1, 6)
+    keys[:, 0] %= 32
+    keys[:, 1] %= 4  # long shared prefixes: deep layer chains that later inserts split
+    _, first = np.unique(keys, axis=0, return_index=True)
+    keys, lens = keys[np.sort(first)], lens[np.sort(first)]
+    vl = rng.integers(0, 4, size=len(lens)).astype(np.uint32)
+    vals = rng.integers(0, 1 << 32, size=(len(lens), 3), dtype=np.uint64).astype(np.uint32)
+    mk, mv = keys.shape[1], 3
+    for hinted in (False, True):
+        b = S.CpuModelBackend(nthreads=nthreads, no_suffix=True, hinted=hinted)
+        o = S.OracleBackend()
+        half = len(lens) // 2
+        _eq = np.array_equal
+        assert _eq(b.insert(keys[:half], lens[:half], vals[:half], vl[:half], mk, mv),
+                   o.insert(keys[:half], lens[:half], vals[:half], vl[:half], mk, mv))
+        b.m.set_no_suffix(False)  # the rest with suffix records, over the layer chains
+        assert _eq(b.insert(keys, lens, vals, vl, mk, mv), o.insert(keys, lens, vals, vl, mk, mv))
+        for g, e in zip(b.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+            assert _eq(g, e)
+        e = rng.choice(len(lens), len(lens) // 3, replace=False)
+        assert _eq(b.erase(keys[e], lens[e], mk), o.erase(keys[e], lens[e], mk))
+        b.m.set_no_suffix(True)
+        assert _eq(b.insert(keys[e], lens[e], vals[e], vl[e], mk, mv, upsert=True),
+                   o.insert(keys[e], lens[e], vals[e], vl[e], mk, mv, upsert=True))
+        assert b.items() == o.items()

@@ -160,6 +160,50 @@ def test_empty_batches_and_tiny_pool_error():
     assert g.t.device_error() == 1  # pool exhausted is reported, not a hang
 
 
+def test_tiny_pool_keeps_working_after_exhaustion():
+    """A pool of 512 slabs runs out during an insert batch: some inserts report false,
+    splits may leave right halves without a parent slot, records whose slabs ran out part
+    way are freed.  The index stays valid and usable: the successful inserts equal the
+    oracle fed only them, finds of every key agree, erases return slabs, and the failed
+    keys then go in."""
+    g, o = S.GpuBackend(16, pool_bytes=1 << 16), S.OracleBackend()
+    rng = np.random.default_rng(3)
+    keys, lens = W.unique_random_keys(rng, 6000, 2, 3)
+    keys[:, 0] %= 64  # shared first slices: suffix records, push-downs, layers
+    _, first = np.unique(keys, axis=0, return_index=True)
+    keys, lens = keys[np.sort(first)], lens[np.sort(first)]
+    vals = rng.integers(0, 1 << 32, size=(len(lens), 2), dtype=np.uint64).astype(np.uint32)
+    mk, mv = 3, 2
+    n = len(lens)
+    r = g._zeros(n, np.uint8)
+    g.t.insert(g._d(keys), mk, g._d(lens), g._d(vals), mv, None, n, results=r)
+    ok = g._h(r, np.uint8)[:n].astype(bool)
+    assert g.t.device_error() == 1  # out of slabs
+    assert 0 < ok.sum() < n
+    idx = np.nonzero(ok)[0]
+    o.insert(keys[idx], lens[idx], vals[idx], None, mk, mv)
+    good, *_, msg = g.validate()
+    assert good, msg
+    assert g.items() == o.items()
+    for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        _eq(a, b)
+    # erase most of what is stored (records go back to the pool), then insert the rest
+    er = idx[: max(1, len(idx) * 3 // 4)]
+    _eq(g.erase(keys[er], lens[er], mk), o.erase(keys[er], lens[er], mk))
+    rest = np.nonzero(~ok)[0][: len(er) // 2]
+    r2 = g._zeros(len(rest), np.uint8)
+    g.t.insert(g._d(keys[rest]), mk, g._d(lens[rest]), g._d(vals[rest]), mv, None, len(rest),
+               results=r2)
+    ok2 = g._h(r2, np.uint8)[:len(rest)].astype(bool)
+    g.t.device_error()  # may run out again; results say which went in
+    assert ok2.sum() > 0
+    idx2 = rest[ok2]
+    o.insert(keys[idx2], lens[idx2], vals[idx2], None, mk, mv)
+    good, *_, msg = g.validate()
+    assert good, msg
+    assert g.items() == o.items()
+
+
 @pytest.mark.parametrize("order", ["ascending", "descending", "two_ranges"])
 def test_key_ordered_insert_batches(order):
     """Insert batches in key order (and one whose second half lies past everything in
