@@ -1,11 +1,13 @@
 """Benchmark of the MI355X Masstree batch path.
 
-Workload (plain GPU batch path, one GPU): 16,777,216 unique keys of 2..8 u32 slices
-(uniform length, uniform random slices), 2-slice values (FNV-1a of the key), inserted
-with one batch; a step is one find batch of 16,777,216 keys (50% inserted keys, 50%
-fresh keys) with inputs resident in HBM.  Also measured once: the insert batch, one
-erase batch of 8,388,608 inserted keys, and (with --mixed) mixed batches over 64M
-preloaded 4-slice keys.  Prints one JSON line.
+Workload of `value` (plain GPU batch path, one GPU): 67,108,864 unique keys of 2..8 u32
+slices (uniform length, uniform random slices), 2-slice values (FNV-1a of the key),
+inserted with one batch; a step is one find batch of 67,108,864 keys (50% inserted keys,
+50% fresh keys) with inputs resident in HBM.  Also measured once, in the same run and on
+the same line: the insert batch, one erase batch of half the keys, mixed batches over 64M
+preloaded 4-slice keys with range scans, the batched form on 16-slice prefixed keys, and
+the served form on a 512M-key resident index (each can be skipped with --skip).  Prints
+one JSON line.
 
 Every find result of the timed batches is checked on the host against the values the
 generator defines (hit: 2 slices = FNV-derived value; miss: length 0), a size-
@@ -29,8 +31,11 @@ HBM_PEAK_GBS = 8000.0
 LINE = 64  # bytes per random HBM line in the roofline's unit
 
 
+T_START = time.time()
+
+
 def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print(f"[{time.time() - T_START:7.1f}s]", *a, file=sys.stderr, flush=True)
 
 
 def up(torch, a, dtype=np.uint32):
@@ -84,34 +89,48 @@ def random_line_ceiling(torch, log2_lines=28, blocks=8192, lines_per_stream=64, 
                 buffer_gib=(1 << log2_lines) * 64 / (1 << 30), ms_per_launch=ms)
 
 
-def cpu_baseline(n_keys, seed, threads):
-    """CPU baselines on a bounded sample of the same workload shape: the oracle (std::map
-    replay, one thread) and the multi-threaded CPU index (std::map shards behind the
-    reference CPU adapter's per-key shape, `threads` workers), each timed for the insert
-    batch and the find batch; the CPU index's finds are checked against the oracle's."""
+def cpu_baseline(n_keys, seed, threads, oracle_sample):
+    """CPU baselines of the same workload: the multi-threaded CPU index (std::map shards
+    behind the reference CPU adapter's per-key shape, `threads` workers) at the full key
+    count, timed for the insert batch and the find batch, its finds checked against the
+    generator's values; and the oracle (std::map replay, one thread) on a bounded sample
+    of `oracle_sample` keys of the same shape, with the CPU index's finds on that sample
+    checked against it."""
     from indexingpu_amd import workloads as W
     from oracle.cpu_index import CpuIndex
     from oracle.oracle import Oracle
     d = W.plain_batch_workload(n_keys, seed=seed)
     mk, mv = d["max_key_length"], d["max_value_length"]
-    keys, vals = d["keys"].reshape(-1).copy(), d["values"].reshape(-1).copy()
-    fkeys = d["find_keys"].reshape(-1).copy()
-    o = Oracle()
-    t0 = time.perf_counter()
-    o.insert(keys, mk, d["lens"], vals, mv, None, n_keys, want_results=False)
-    t1 = time.perf_counter()
-    ov, ol = o.find(fkeys, mk, d["find_lens"], mv, n_keys)
-    t2 = time.perf_counter()
+    keys, vals = d["keys"].reshape(-1), d["values"].reshape(-1)
+    fkeys = d["find_keys"].reshape(-1)
     c = CpuIndex(shard_bits=10, threads=threads)
-    t3 = time.perf_counter()
+    t0 = time.perf_counter()
     c.insert(keys, mk, d["lens"], vals, mv, None, n_keys)
-    t4 = time.perf_counter()
+    t1 = time.perf_counter()
     cv, cl = c.find(fkeys, mk, d["find_lens"], mv, n_keys)
+    t2 = time.perf_counter()
+    hit = cl > 0
+    exp_v = W.make_values(d["find_keys"], d["find_lens"], mv)
+    full_ok = bool(int(hit.sum()) == n_keys // 2 and (cl[hit] == mv).all()
+                   and (cv.reshape(n_keys, mv)[hit] == exp_v[hit]).all())
+    del c, cv, cl, exp_v, d, keys, vals, fkeys
+    s = W.plain_batch_workload(oracle_sample, seed=seed)
+    sk, sv = s["keys"].reshape(-1).copy(), s["values"].reshape(-1).copy()
+    sf = s["find_keys"].reshape(-1).copy()
+    o = Oracle()
+    t3 = time.perf_counter()
+    o.insert(sk, mk, s["lens"], sv, mv, None, oracle_sample, want_results=False)
+    t4 = time.perf_counter()
+    ov, ol = o.find(sf, mk, s["find_lens"], mv, oracle_sample)
     t5 = time.perf_counter()
-    same = bool(np.array_equal(cl, ol) and np.array_equal(cv, ov))
-    return dict(find_mops=n_keys / (t2 - t1) / 1e6, insert_mops=n_keys / (t1 - t0) / 1e6,
-                mt_find_mops=n_keys / (t5 - t4) / 1e6, mt_insert_mops=n_keys / (t4 - t3) / 1e6,
-                mt_same=same, seconds=t5 - t0)
+    c2 = CpuIndex(shard_bits=10, threads=threads)
+    c2.insert(sk, mk, s["lens"], sv, mv, None, oracle_sample)
+    c2v, c2l = c2.find(sf, mk, s["find_lens"], mv, oracle_sample)
+    same = bool(np.array_equal(c2l, ol) and np.array_equal(c2v, ov))
+    return dict(mt_find_mops=n_keys / (t2 - t1) / 1e6, mt_insert_mops=n_keys / (t1 - t0) / 1e6,
+                mt_full_ok=full_ok, mt_seconds=t2 - t0,
+                find_mops=oracle_sample / (t5 - t4) / 1e6,
+                insert_mops=oracle_sample / (t4 - t3) / 1e6, mt_same=same)
 
 
 def run_mixed(torch, args):
@@ -385,8 +404,9 @@ def run_served(torch, args):
     arrivals at a sweep of rates.  The served loop (continuous batching: waves side by side
     on 4 streams, one validated launch each; or, as served_ordered, the point ops of all
     waves in order on one stream as a read-only pass then the writes) and the plain path
-    (one mixed-batch call plus one scan call per request, in order) replay traces of the
-    same kind; each point reports Mops/s (operations of
+    (the index's mixed-batch and scan calls on one stream: plain_batched, one call pair
+    over every request that arrived while the previous pair ran; plain, one pair per
+    request) replay traces of the same kind; each point reports Mops/s (operations of
     completed requests over the run's wall time) and p50/p99/p99.9 latency from arrival
     to completion.  Point-op results are checked against the generator's expectation;
     scan counts are checked (every scan lies in a region with >= 50 keys after it, except
@@ -410,15 +430,17 @@ def run_served(torch, args):
     modes = [m for m in args.served_modes.split(",") if m]
     points = {m: [] for m in modes}
     for mode in modes:
+        plain = mode.startswith("plain")
         sv = MasstreeServer(tree, streams=4, max_wave_ops=args.served_wave_ops,
                             max_wave_scans=4096, ring_rows=1 << 22, epoch_waves=16,
-                            coalesce=(mode != "plain"), ordered=(mode == "served_ordered"))
+                            coalesce=not plain, ordered=(mode == "served_ordered"),
+                            plain_batch=(mode == "plain_batched"))
         for rate in rates:
             nreq = int(min(args.served_max_requests, max(2000, rate * args.served_seconds)))
             arena, req, expect = wl.trace(nreq, rate)
             mc = arena["scan_max_count"]
             n_scans = int(req["scan_offset"][-1] + req["num_scans"][-1])
-            arena = D.ServeWorkload.with_ring(arena, (1 << 22) if mode != "plain" else n_scans * mc)
+            arena = D.ServeWorkload.with_ring(arena, n_scans * mc if plain else (1 << 22))
             out, st = sv.run(arena, req)
             tree.check_device_error()
             ok = all(torch.equal(arena[key].view(-1), expect[key].reshape(-1).to(arena[key].dtype))
@@ -447,15 +469,20 @@ def run_served(torch, args):
         return max(good, key=lambda p: p["mops"]) if good else None
     bs, bp = best(points.get("served", [])), best(points.get("plain", []))
     bso = best(points.get("served_ordered", []))
+    bpb = best(points.get("plain_batched", []))
     # the served form's best point at p99 <= 1 ms over its two wave modes
     both = [p for p in (bs, bso) if p]
     top = max(both, key=lambda p: p["mops"]) if both else None
+
+    def ratio(a, b):
+        return round(a["mops"] / b["mops"], 2) if a and b else None
     return dict(served=dict(resident_keys=n, preload_mops=round(pre, 2), points=points,
                             best_served_at_p99_1ms=bs, best_ordered_at_p99_1ms=bso,
-                            best_plain_at_p99_1ms=bp,
-                            ratio=(round(bs["mops"] / bp["mops"], 2) if bs and bp else None),
-                            ratio_best_mode=(round(top["mops"] / bp["mops"], 2)
-                                             if top and bp else None)))
+                            best_plain_at_p99_1ms=bp, best_plain_batched_at_p99_1ms=bpb,
+                            # against the interval-batched plain path (the honest baseline)
+                            ratio_vs_plain_batched=ratio(top, bpb),
+                            # against one plain call pair per request
+                            ratio_vs_plain_per_request=ratio(top, bp)))
 
 
 def main():
@@ -463,14 +490,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--keys", type=int, default=1 << 24)
+    ap.add_argument("--keys", type=int, default=1 << 26)
     ap.add_argument("--tile", type=int, default=16)
-    ap.add_argument("--pool-gb", type=float, default=24.0)
+    ap.add_argument("--pool-gb", type=float, default=48.0)
+    ap.add_argument("--skip", default="",
+                    help="comma-separated parts not to run: mixed, batched, served, cpu, "
+                         "ceiling")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ceiling", action="store_true",
                     help="skip the random-line read ceiling measurement")
-    ap.add_argument("--mixed", action="store_true", help="also run 64M-key mixed batches")
+    ap.add_argument("--mixed", action="store_true", help="(default: on) 64M-key mixed batches")
     ap.add_argument("--mixed-keys", type=int, default=1 << 26)
     ap.add_argument("--mixed-batch", type=int, default=1 << 20)
     ap.add_argument("--mixed-steps", type=int, default=8)
@@ -478,14 +508,14 @@ def main():
     ap.add_argument("--mixed-pool-gb", type=float, default=48.0)
     ap.add_argument("--scan-queries", type=int, default=65536)
     ap.add_argument("--batched", action="store_true",
-                    help="also run the batched-form comparison (16-slice prefixed keys)")
+                    help="(default: on) the batched-form comparison (16-slice prefixed keys)")
     ap.add_argument("--batched-keys", type=int, default=1 << 26)
     ap.add_argument("--batched-count", type=int, default=2000)
     ap.add_argument("--batched-pool-gb", type=float, default=32.0)
     ap.add_argument("--batched-tune-unit", type=int, default=16,
                     help="batches per autotuner measurement unit")
     ap.add_argument("--served", action="store_true",
-                    help="also run the served form against the plain path (rate sweep)")
+                    help="(default: on) the served form against the plain path (rate sweep)")
     ap.add_argument("--served-keys", type=int, default=1 << 29)
     ap.add_argument("--served-pool-gb", type=float, default=112.0)
     ap.add_argument("--served-rates",
@@ -493,10 +523,12 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=0.25)
     ap.add_argument("--served-max-requests", type=int, default=1_000_000)
     ap.add_argument("--served-wave-ops", type=int, default=1 << 18)
-    ap.add_argument("--served-modes", default="served,served_ordered,plain",
+    ap.add_argument("--served-modes", default="served,served_ordered,plain_batched,plain",
                     help="served (waves side by side on 4 streams, one validated launch "
                          "each), served_ordered (point ops of all waves in order on one "
-                         "stream, read-only passes), plain (one mixed-batch + scan call "
+                         "stream, read-only passes), plain_batched (one mixed-batch + scan "
+                         "call pair over every request that arrived while the previous pair "
+                         "ran), plain (one mixed-batch + scan call "
                          "per request)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a kernel of "
@@ -506,6 +538,9 @@ def main():
                     help="passes,find_tile,depth[,record_tile]: pin the read-only find "
                          "configuration instead of tuning it (for profiling runs)")
     args = ap.parse_args()
+    skip = set(x for x in args.skip.split(",") if x)
+    args.no_cpu_baseline = args.no_cpu_baseline or "cpu" in skip
+    args.no_ceiling = args.no_ceiling or "ceiling" in skip
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: every box in this pool has one GPU, so "
               f"--gpus {args.gpus} is not supported (the index shards by key range across "
@@ -607,7 +642,9 @@ def main():
     # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated by a 1 GiB copy; one
     # entry per kernel, every configuration the tuner tries); null for another workload
     traffic, traffic_note = None, None
-    tpath = os.path.join(ROOT, "profiles", "find_kernel_16M_varlen_traffic.json")
+    tname = {1 << 24: "find_kernel_16M_varlen_traffic.json",
+             1 << 26: "find_kernel_64M_varlen_traffic.json"}.get(n, "")
+    tpath = os.path.join(ROOT, "profiles", tname) if tname else ""
     if best.get("find_passes", 1) == 2:
         rtile = best.get("record_tile", 0) or best["find_tile"]
         parts = [f"narrow_descend_kernel<{best['find_tile']}, {best['find_depth']}>",
@@ -615,12 +652,12 @@ def main():
     else:
         parts = [f"narrow_find_kernel<{best['find_tile']}, {best['find_depth']}>"]
     kname = " + ".join(parts)
-    if os.path.exists(tpath) and n == (1 << 24) and args.seed == 2:
+    if tpath and os.path.exists(tpath) and args.seed == 2:
         t = json.load(open(tpath))
         ks = t.get("kernels", {})
         if all(p in ks for p in parts):
             traffic = sum(ks[p]["bytes"] for p in parts)
-            traffic_note = ("profiles/find_kernel_16M_varlen_traffic.json: FETCH_SIZE x 1024 x "
+            traffic_note = (f"profiles/{tname}: FETCH_SIZE x 1024 x "
                             f"{t['read_scale']} + WRITE_SIZE x 1024 x {t['write_scale']}, "
                             "summed over the kernels of a find batch")
     result = {
@@ -637,8 +674,8 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": "plain batch path: 16M unique var-len keys (2..8 u32 slices), "
-                        "2-slice values; step = one find batch of all keys' count "
+            "workload": f"plain batch path: {n} unique var-len keys (2..8 u32 slices), "
+                        "2-slice values; step = one find batch of as many keys "
                         "(50% hits) on the loaded index",
             "keys": n, "find_batch": n, "erase_batch": n_erase, "max_key_length": mk,
             "max_value_length": mv, "tile_size": args.tile,
@@ -674,28 +711,32 @@ def main():
         result["roofline"]["measured_ceiling_gbs"] = round(ceil["gbs"], 1)
         result["roofline"]["frac_of_measured_ceiling"] = round(achieved / ceil["gbs"], 4)
         result["random_line_ceiling"] = {k: round(v, 3) for k, v in ceil.items()}
-    if args.mixed:
+    if "mixed" not in skip:
         result.update(run_mixed(torch, args))
-    if args.batched:
+    if "batched" not in skip:
         result.update(run_batched(torch, args))
-    if args.served:
+    if "served" not in skip:
         result.update(run_served(torch, args))
 
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
-        log(f"[bench] CPU baselines on {args.cpu_sample} keys ({threads} threads) ...")
-        cb = cpu_baseline(args.cpu_sample, seed=args.seed, threads=threads)
+        log(f"[bench] CPU baselines: {n} keys ({threads} threads), oracle on "
+            f"{args.cpu_sample} keys ...")
+        cb = cpu_baseline(n, args.seed, threads, args.cpu_sample)
         result["cpu_baseline"] = {
             "value": round(cb["mt_find_mops"], 3), "unit": "Mops/s", "cores": threads,
             "kind": "port",
-            "sample": f"{args.cpu_sample} inserts then {args.cpu_sample} finds (50% hits) of "
-                      f"the same workload shape. value = finds of the multi-threaded CPU index "
-                      f"(oracle/cpu_index.cpp: std::map shards behind the reference CPU "
-                      f"adapter's per-key shape, not masstree-beta; {threads} threads; "
-                      f"insert {cb['mt_insert_mops']:.2f} Mops/s; results equal the oracle's: "
-                      f"{cb['mt_same']}). Oracle std::map replay, 1 thread: find "
-                      f"{cb['find_mops']:.3f}, insert {cb['insert_mops']:.3f} Mops/s. Host "
-                      f"{platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs",
+            "sample": f"the same workload at full size: {n} inserts then one find batch of {n} "
+                      f"keys (50% hits), {cb['mt_seconds']:.1f} s. value = finds of the "
+                      f"multi-threaded CPU index (oracle/cpu_index.cpp: std::map shards behind "
+                      f"the reference CPU adapter's per-key shape, not masstree-beta; "
+                      f"{threads} threads; insert {cb['mt_insert_mops']:.2f} Mops/s; finds "
+                      f"equal the generator's values: {cb['mt_full_ok']}; on a "
+                      f"{args.cpu_sample}-key sample they equal the oracle's: {cb['mt_same']}). "
+                      f"Oracle std::map replay, 1 thread, on the {args.cpu_sample}-key sample: "
+                      f"find {cb['find_mops']:.3f}, insert {cb['insert_mops']:.3f} Mops/s. "
+                      f"Host {platform.processor() or platform.machine()}, {os.cpu_count()} "
+                      f"logical CPUs",
         }
     print(json.dumps(result), flush=True)
 

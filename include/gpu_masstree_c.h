@@ -202,8 +202,8 @@ uint32_t gmt_tile_size(gmt_index* index);
  * `streams` HIP streams, so waves overlap on the device; with ordered = 1 the point ops
  * of up to `streams` waves queue in order on one stream, each wave a read-only pass and
  * then its writes, and its scans run beside them on the wave's own stream.  coalesce = 0 runs the plain path instead
- * (one request per gmt_mixed_batch / gmt_scan pair, in order, on one stream) for
- * comparison. */
+ * (gmt_mixed_batch / gmt_scan pairs in order on one stream: one request per pair, or with
+ * plain_batch = 1 everything that arrived while the previous pair ran) for comparison. */
 typedef struct gmt_server gmt_server;
 
 typedef struct {
@@ -218,6 +218,10 @@ typedef struct {
                                 each wave a read-only pass (finds, write start hints) then
                                 its writes, its scans beside them on the wave's stream;
                                 0 (recommended): waves side by side on `streams` streams */
+  uint32_t plain_batch;      /* with coalesce = 0: 1 = each gmt_mixed_batch / gmt_scan pair
+                                covers every request that has arrived (up to the per-wave
+                                limits), one pair at a time (interval batching of the plain
+                                path); 0 = one request per pair */
 } gmt_serve_config;
 
 /* Device arena of a trace: point ops in gmt_mixed_batch's layout, scans in gmt_scan's

@@ -18,6 +18,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "device_context.hpp"
 #include "hip_util.hpp"
 #include "host_walk.hpp"
 #include "kernels.hpp"
@@ -39,34 +40,6 @@ enum request_type : uint8_t {
 }  // namespace kernels
 
 namespace indexingpu_amd {
-
-// Device-side contexts (the reference's device_allocator_context<Alloc> /
-// device_reclaimer_context<Reclaimer>, include/device_context.hpp:18-21).  They carry a
-// tile's cached bitmap word and retire reservation between cooperative calls; a tile
-// returns what is left when its context is destroyed.
-template <class Alloc>
-struct device_allocator_context {
-  slab_pool_view view;
-  tile_slab_cache cache;
-  template <class Tile>
-  __device__ device_allocator_context(const slab_pool_view& v, const Tile&) : view(v) {}
-  __device__ ~device_allocator_context() {
-    if (cache.bits != 0 && (__lane_id() & 15u) == 0)
-      dev::atomic_and(&view.bitmap[cache.word], ~cache.bits);
-  }
-};
-
-template <class Reclaimer>
-struct device_reclaimer_context {
-  retire_view view;
-  tile_retire_cache cache;
-  template <class Tile>
-  __device__ device_reclaimer_context(const retire_view& v, const Tile&) : view(v) {}
-  __device__ ~device_reclaimer_context() {
-    if (cache.base != kInvalid && (__lane_id() & 15u) == 0)
-      for (u32 j = cache.used; j < kRetireChunk; j++) view.list[cache.base + j] = kInvalid;
-  }
-};
 
 inline u32 device_cu_count() {
   int dev = 0, cus = 0;
@@ -95,8 +68,8 @@ inline u32 resident_blocks(K kernel) {
 
 namespace GpuMasstree {
 
-using indexingpu_amd::device_allocator_context;
-using indexingpu_amd::device_reclaimer_context;
+using ::device_allocator_context;
+using ::device_reclaimer_context;
 
 template <typename Allocator, typename Reclaimer, uint32_t tile_size = 16>
 struct gpu_masstree {
@@ -164,6 +137,7 @@ struct gpu_masstree {
             const size_type* key_lengths, value_slice_type* values,
             const size_type max_value_length, size_type* value_lengths,
             const size_type num_keys, hipStream_t stream = 0) {
+    if (!key_length_ok(max_key_length, stream)) return;
     indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
     a.kind = indexingpu_amd::op_find;
     a.max_value_length = max_value_length;
@@ -207,9 +181,10 @@ struct gpu_masstree {
               const size_type* key_lengths, const value_slice_type* values,
               const size_type max_value_length, const size_type* value_lengths,
               const size_type num_keys, hipStream_t stream = 0, bool* results = nullptr) {
-    static_assert(enable_suffix, "layers without suffix records are not provided");
+    if (!key_length_ok(max_key_length, stream)) return;
     indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
     a.kind = update_if_exists ? indexingpu_amd::op_upsert : indexingpu_amd::op_insert;
+    a.flags = enable_suffix ? 0u : indexingpu_amd::kReqNoSuffix;
     a.values = values;
     a.max_value_length = max_value_length;
     a.value_lengths = value_lengths;
@@ -222,6 +197,7 @@ struct gpu_masstree {
               const size_type* key_lengths, const value_slice_type* values,
               const size_type max_value_length, const size_type* value_lengths,
               const size_type num_keys, hipStream_t stream = 0, bool* results = nullptr) {
+    if (!key_length_ok(max_key_length, stream)) return;
     indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
     a.kind = indexingpu_amd::op_update;
     a.values = values;
@@ -231,15 +207,19 @@ struct gpu_masstree {
     launch_hinted(a, stream);
   }
 
-  // Erase never merges nodes here (see tree_ops.hpp), so the merge switches of the
-  // reference's signature are accepted and have no effect.
+  // Erase removes layer roots it leaves empty (do_remove_empty_root, as the reference);
+  // it does not merge or borrow between nodes of a layer, so do_merge and
+  // pessimistic_merge are accepted and have no effect on structure (results are the
+  // same either way).
   template <bool do_remove_empty_root = true, bool pessimistic_merge = true, bool do_merge = true,
             bool concurrent = true, bool use_shmem_key = false>
   void erase(const key_slice_type* keys, const size_type max_key_length,
              const size_type* key_lengths, const size_type num_keys, hipStream_t stream = 0,
              bool* results = nullptr) {
+    if (!key_length_ok(max_key_length, stream)) return;
     indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
     a.kind = indexingpu_amd::op_erase;
+    a.flags = do_remove_empty_root ? 0u : indexingpu_amd::kReqKeepEmptyLayers;
     a.results = results;
     launch_hinted(a, stream);
   }
@@ -253,7 +233,7 @@ struct gpu_masstree {
             size_type* value_lengths = nullptr, key_slice_type* out_keys = nullptr,
             size_type* out_key_lengths = nullptr, hipStream_t stream = 0,
             const size_type* per_query_limits = nullptr) {
-    if (num_queries == 0) return;
+    if (num_queries == 0 || !key_length_ok(max_key_length, stream)) return;
     indexingpu_amd::scan_args a{};
     a.lower_keys = lower_keys;
     a.lower_key_lengths = lower_key_lengths;
@@ -288,8 +268,11 @@ struct gpu_masstree {
                    value_slice_type* values, const size_type max_value_length,
                    size_type* value_lengths, bool* results, const size_type num_requests,
                    hipStream_t stream = 0) {
+    if (!key_length_ok(max_key_length, stream)) return;
     indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_requests);
     a.types = reinterpret_cast<const uint8_t*>(request_types);
+    a.flags = (enable_suffix ? 0u : indexingpu_amd::kReqNoSuffix) |
+              (erase_do_remove_empty_root ? 0u : indexingpu_amd::kReqKeepEmptyLayers);
     a.kind = indexingpu_amd::op_find;
     a.values = values;
     a.max_value_length = max_value_length;
@@ -338,7 +321,7 @@ struct gpu_masstree {
                    size_type* value_lengths, bool* results, const uint64_t* offsets,
                    const size_type num_batches, const uint64_t total_requests,
                    hipStream_t stream = 0) {
-    if (num_batches == 0) return;
+    if (num_batches == 0 || !key_length_ok(max_key_length, stream)) return;
     indexingpu_amd::multi_args m{};
     m.a = base_args(keys, max_key_length, key_lengths, total_requests);
     m.a.types = reinterpret_cast<const uint8_t*>(request_types);
@@ -446,17 +429,66 @@ struct gpu_masstree {
     return run_op<concurrent>(c, rq);
   }
 
+  // Range query from a user kernel (the reference's cooperative_scan,
+  // include/gpu_masstree.hpp:250-395): up to out_max_count entries with key >= the lower
+  // key and, with use_upper_key, <= the upper key, ascending; entry c's value goes to
+  // out_values + c * out_value_max_length (min(len, max) slices) with its full length in
+  // out_value_lengths[c], its key likewise to out_keys / out_key_lengths; any output may
+  // be null.  Returns the number of entries.  The layer stack of keys that span trie
+  // layers takes slabs from the tile's allocator context while the scan runs.
+  template <bool use_upper_key, bool concurrent, typename tile_type, typename K>
+  __device__ size_type cooperative_scan(K& lower_key, const size_type lower_key_length,
+                                        const tile_type& tile,
+                                        device_allocator_context_type& allocator,
+                                        const key_slice_type* upper_key = nullptr,
+                                        const size_type upper_key_length = 1,
+                                        size_type out_max_count = 1,
+                                        value_slice_type* out_values = nullptr,
+                                        size_type* out_value_lengths = nullptr,
+                                        const size_type out_value_max_length = 1,
+                                        key_slice_type* out_keys = nullptr,
+                                        size_type* out_key_lengths = nullptr,
+                                        const size_type out_key_max_length = 1) {
+    indexingpu_amd::wave_tile<tile_size, concurrent> c(allocator.view,
+                                                       indexingpu_amd::retire_view{}, err_);
+    if (out_max_count == 0) return 0;
+    if (lower_key_length == 0 || (use_upper_key && (upper_key == nullptr || upper_key_length == 0))) {
+      c.set_error(indexingpu_amd::kErrBadLength);
+      return 0;
+    }
+    c.ac = allocator.cache;
+    indexingpu_amd::scan_query<tile_size> q;
+    q.lo = as_ptr(lower_key);
+    q.nlo = lower_key_length;
+    q.use_hi = use_upper_key;
+    q.hi = use_upper_key ? upper_key : nullptr;
+    q.nhi = use_upper_key ? upper_key_length : 0u;
+    q.limit = out_max_count;
+    q.vals = out_values;
+    q.vlens = out_value_lengths;
+    q.keys = out_keys;
+    q.klens = out_key_lengths;
+    q.maxV = out_value_max_length;
+    q.maxK = out_key_max_length;
+    indexingpu_amd::slab_scan_stack stk;
+    const size_type n = indexingpu_amd::tile_scan<tile_size, concurrent>(c, root_index_, q, stk);
+    stk.release(c);
+    allocator.cache = c.ac;
+    return n;
+  }
+
   template <bool update_if_exists = false, bool enable_suffix = true, typename tile_type,
             typename K, typename V>
   __device__ bool cooperative_insert(K& key, const size_type key_length, V& value,
                                      const size_type value_length, const tile_type& tile,
                                      device_allocator_context_type& allocator,
                                      device_reclaimer_context_type& reclaimer) {
-    indexingpu_amd::wave_tile<tile_size, true> c(allocator.view, reclaimer.view, err_);
+    indexingpu_amd::wave_tile<tile_size, true> c(allocator.view, reclaimer.view, err_, root_index_);
     c.ac = allocator.cache;
     c.rc = reclaimer.cache;
     indexingpu_amd::request rq{update_if_exists ? indexingpu_amd::op_upsert : indexingpu_amd::op_insert,
-                               key_length, as_ptr(key), as_ptr(value), value_length, nullptr, 0};
+                               key_length, as_ptr(key), as_ptr(value), value_length, nullptr, 0,
+                               enable_suffix ? 0u : indexingpu_amd::kReqNoSuffix};
     const bool r = run_op<true>(c, rq) != 0;
     allocator.cache = c.ac;
     reclaimer.cache = c.rc;
@@ -468,7 +500,7 @@ struct gpu_masstree {
                                      const size_type value_length, const tile_type& tile,
                                      device_allocator_context_type& allocator,
                                      device_reclaimer_context_type& reclaimer) {
-    indexingpu_amd::wave_tile<tile_size, true> c(allocator.view, reclaimer.view, err_);
+    indexingpu_amd::wave_tile<tile_size, true> c(allocator.view, reclaimer.view, err_, root_index_);
     c.ac = allocator.cache;
     c.rc = reclaimer.cache;
     indexingpu_amd::request rq{indexingpu_amd::op_update, key_length, as_ptr(key), as_ptr(value),
@@ -484,11 +516,12 @@ struct gpu_masstree {
   __device__ bool cooperative_erase(K& key, const size_type key_length, const tile_type& tile,
                                     device_allocator_context_type& allocator,
                                     device_reclaimer_context_type& reclaimer) {
-    indexingpu_amd::wave_tile<tile_size, true> c(allocator.view, reclaimer.view, err_);
+    indexingpu_amd::wave_tile<tile_size, true> c(allocator.view, reclaimer.view, err_, root_index_);
     c.ac = allocator.cache;
     c.rc = reclaimer.cache;
     indexingpu_amd::request rq{indexingpu_amd::op_erase, key_length, as_ptr(key), nullptr, 0,
-                               nullptr, 0};
+                               nullptr, 0,
+                               do_remove_empty_root ? 0u : indexingpu_amd::kReqKeepEmptyLayers};
     const bool r = run_op<true>(c, rq) != 0;
     allocator.cache = c.ac;
     reclaimer.cache = c.rc;
@@ -599,6 +632,16 @@ struct gpu_masstree {
 
   indexingpu_amd::tree_view view() const {
     return indexingpu_amd::tree_view{allocator_, reclaimer_, err_, root_index_};
+  }
+
+  // Keys of more than 65,536 slices are refused (the device error word reports
+  // kErrBadLength, the call does nothing): record lengths and trie layers are 16-bit.
+  static constexpr uint32_t kMaxKeyLength = 65536;
+  bool key_length_ok(uint32_t max_key_length, hipStream_t stream) {
+    if (max_key_length <= kMaxKeyLength) return true;
+    hipLaunchKernelGGL(indexingpu_amd::set_error_kernel, dim3(1), dim3(1), 0, stream, err_,
+                       (uint32_t)indexingpu_amd::kErrBadLength);
+    return false;
   }
 
   static indexingpu_amd::batch_args base_args(const uint32_t* keys, uint32_t maxK,

@@ -68,8 +68,9 @@ struct batch_args {
   // none).  The write launch then starts each write at its border node instead of the
   // root.
   u32* hints;
-  // insert / upsert with enable_suffix = false (layers instead of suffix records)
-  u32 no_suffix;
+  // kReqNoSuffix (insert / upsert with enable_suffix = false), kReqKeepEmptyLayers
+  // (erase<do_remove_empty_root = false>)
+  u32 flags;
 };
 constexpr u32 kHintWords = 4;
 
@@ -86,7 +87,7 @@ __device__ __forceinline__ void load_request(const batch_args& a, u64 i, request
   rq.value_length = a.value_lengths ? a.value_lengths[i] : a.max_value_length;
   rq.out_value = a.out_values ? a.out_values + i * (u64)a.max_value_length : nullptr;
   rq.max_value_length = a.out_values ? a.max_value_length : 0u;
-  rq.no_suffix = a.no_suffix;
+  rq.flags = a.flags;
 }
 
 // After a corruption or livelock guard fired, the remaining requests of the launch are
@@ -102,7 +103,7 @@ __device__ __forceinline__ bool abort_pending(u32* err) {
 template <u32 T, bool Validate>
 __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6)))
 batch_kernel(tree_view tv, batch_args a) {
-  wave_tile<T, Validate> c(tv.pool, tv.retire, tv.err);
+  wave_tile<T, Validate> c(tv.pool, tv.retire, tv.err, tv.root);
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
   u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
@@ -237,7 +238,7 @@ constexpr u64 kBarrierTicks = 20ull * 100000000ull;   // 20 s at the 100 MHz wal
 template <u32 T>
 __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5)))
 multi_batch_kernel(tree_view tv, multi_args m) {
-  wave_tile<T, true> c(tv.pool, tv.retire, tv.err);
+  wave_tile<T, true> c(tv.pool, tv.retire, tv.err, tv.root);
   const batch_args& a = m.a;
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
@@ -409,8 +410,66 @@ struct scan_query {
   u32* klens;
   u32 maxV;
   u32 maxK;
-  u32* stack;
-  u32 stack_layers;
+};
+
+// Layer stack of a scan: 3 words per trie layer entered (slice of the LINK entry,
+// border node holding it, upper-bound flag); the key prefix of an emitted entry is the
+// stacked slices.  Every lane of the tile calls these (uniform arguments); lane 0
+// writes.
+// Flat form: the tile's region of a scratch buffer (scan_kernel).
+struct flat_scan_stack {
+  u32* base;
+  u32 layers;
+  template <class C>
+  __device__ __forceinline__ bool reserve(C&, u32 l) { return l + 1 < layers; }
+  __device__ __forceinline__ u32* at(u32 l) const { return base + 3u * l; }
+  template <class C>
+  __device__ __forceinline__ void release(C&) {}
+};
+// Slab form (device API): 10 layers per 128-byte slab taken from the tile's allocator,
+// word 30 of a slab links the next; slabs are taken as the scan goes deeper and freed
+// when it ends.  Nothing else ever sees them.
+struct slab_scan_stack {
+  static constexpr u32 kPer = 10, kNext = 30;
+  u64* pool = nullptr;
+  u32 head = kInvalid;
+  u32 slabs = 0;
+  __device__ __forceinline__ u32* slab_words(u32 idx) const {
+    return reinterpret_cast<u32*>(pool + (u64)idx * kSlots);
+  }
+  __device__ __forceinline__ u32 nth(u32 k) const {
+    u32 idx = head;
+    for (u32 i = 0; i < k; i++) idx = slab_words(idx)[kNext];
+    return idx;
+  }
+  template <class C>
+  __device__ __forceinline__ bool reserve(C& c, u32 l) {
+    pool = c.pool.pool;
+    while (l / kPer >= slabs) {
+      const u32 s = c.alloc_slab();
+      if (s == kInvalid) return false;
+      if (c.lane == 0) {
+        if (slabs == 0) head = s;
+        else slab_words(nth(slabs - 1))[kNext] = s;
+      }
+      head = c.bcast(head, 0);
+      slabs += 1;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+    return true;
+  }
+  __device__ __forceinline__ u32* at(u32 l) const { return slab_words(nth(l / kPer)) + 3u * (l % kPer); }
+  template <class C>
+  __device__ __forceinline__ void release(C& c) {
+    u32 idx = head;
+    for (u32 k = 0; k < slabs; k++) {
+      const u32 nxt = slab_words(idx)[kNext];
+      if (c.lane == 0) dev::free_slab(c.pool, idx);
+      idx = nxt;
+    }
+    slabs = 0;
+    head = kInvalid;
+  }
 };
 
 // number of entries e with (e.slice, e.more) <= (s, m)
@@ -468,8 +527,8 @@ __device__ __forceinline__ void scan_read(wave_tile<T, C>& c, u32 idx, gpu_node&
 }
 
 // Returns the number of results.
-template <u32 T, bool Validate, bool C>
-__device__ u32 tile_scan(wave_tile<T, C>& c, u32 root, const scan_query<T>& q) {
+template <u32 T, bool Validate, bool C, class Stack>
+__device__ u32 tile_scan(wave_tile<T, C>& c, u32 root, const scan_query<T>& q, Stack& stk) {
   if (q.limit == 0) return 0;
   u32 count = 0;
   u32 l = 0;
@@ -562,7 +621,7 @@ __device__ u32 tile_scan(wave_tile<T, C>& c, u32 root, const scan_query<T>& q) {
           const u32 o = count + (e - p);
           if (q.keys) {
             u32* dst = q.keys + (u64)o * q.maxK;
-            for (u32 j = 0; j <= l && j < q.maxK; j++) dst[j] = (j == l) ? slice : q.stack[3 * j];
+            for (u32 j = 0; j <= l && j < q.maxK; j++) dst[j] = (j == l) ? slice : stk.at(j)[0];
           }
           if (ks == kValue) {
             if (q.vlens) q.vlens[o] = 1;
@@ -628,16 +687,17 @@ __device__ u32 tile_scan(wave_tile<T, C>& c, u32 root, const scan_query<T>& q) {
       if (count >= q.limit || stop_after) return count;
       if (lk < end) {
         // descend into the next layer
-        if (l + 1 >= q.stack_layers) {
+        if (!stk.reserve(c, l)) {
           c.set_error(kErrScanScratch);
           return count;
         }
         const u32 slice = c.bcast(slot_lo(n.s), lk);
         const u32 child = c.bcast(slot_hi(n.s), lk);
+        u32* top = stk.at(l);
         if (c.lane == 0) {
-          q.stack[3 * l] = slice;
-          q.stack[3 * l + 1] = node;
-          q.stack[3 * l + 2] = hi ? 1u : 0u;
+          top[0] = slice;
+          top[1] = node;
+          top[2] = hi ? 1u : 0u;
         }
         // other lanes of the tile read the stack later (key prefixes, pops)
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -666,9 +726,10 @@ __device__ u32 tile_scan(wave_tile<T, C>& c, u32 root, const scan_query<T>& q) {
     if (l == 0) return count;
     l -= 1;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    rs = q.stack[3 * l];
-    node = q.stack[3 * l + 1];
-    hi = q.stack[3 * l + 2] != 0;
+    const u32* top = stk.at(l);
+    rs = top[0];
+    node = top[1];
+    hi = top[2] != 0;
     lo = false;
     resume = true;
   }
@@ -699,13 +760,12 @@ __global__ void __launch_bounds__(kBlockThreads) scan_kernel(tree_view tv, scan_
     q.klens = a.out_key_lengths ? a.out_key_lengths + slot0 : nullptr;
     q.maxV = a.max_value_length;
     q.maxK = a.max_key_length;
-    q.stack = a.stack + gtile * (u64)a.stack_layers * 3u;
-    q.stack_layers = a.stack_layers;
+    flat_scan_stack stk{a.stack + gtile * (u64)a.stack_layers * 3u, a.stack_layers};
     u32 cnt = 0;
     if (q.nlo == 0 || q.nlo > a.max_key_length || (q.use_hi && (q.nhi == 0 || q.nhi > a.max_key_length))) {
       c.set_error(kErrBadLength);
     } else {
-      cnt = tile_scan<T, Validate>(c, tv.root, q);
+      cnt = tile_scan<T, Validate>(c, tv.root, q, stk);
     }
     if (c.lane == 0 && a.counts) a.counts[i] = cnt;
   }
@@ -719,6 +779,8 @@ __global__ void drain_kernel(slab_pool_view pool, retire_view rv) {
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
     dev::free_retired(pool, rv.list[i]);
 }
+
+__global__ void set_error_kernel(u32* err, u32 code) { dev::set_error(err, code); }
 
 __global__ void reset_kernel(retire_view rv) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *rv.count = 0;

@@ -16,7 +16,8 @@
 // first i with q <= bound[i]; no such i means the node split and q lives to the
 // right (B-link move-right).
 //
-// Header word: nkeys:5 | border:1 | root:1 | lock:1 | level:4 | version:20.
+// Header word: nkeys:5 | border:1 | root:1 | lock:1 | level:4 | version:20.  Level 15
+// marks a layer root that an erase removed (hdr_removed).
 // Writers set `lock` with an atomic fetch_or, rewrite the node, then release-store
 // the header with `lock` cleared and `version` advanced; optimistic readers accept a
 // node image only if its header was unlocked and is unchanged when re-read.
@@ -84,6 +85,12 @@ IGPU_HD u32 hdr_make(u32 nkeys, bool border, bool root, u32 level) {
 }
 // header published at unlock: lock cleared, version advanced
 IGPU_HD u32 hdr_unlocked_next(u32 h) { return (h & ~kLockBit) + kVersionOne; }
+// A removed layer root (an erase emptied it and unlinked it) carries level 15, which no
+// live node reaches (15 levels of at least 7 children each exceed any pool).
+constexpr u32 kRemovedLevel = 15;
+IGPU_HD bool hdr_removed(u32 h) { return hdr_level(h) == kRemovedLevel; }
+IGPU_HD u32 hdr_remove(u32 h) { return (h & ~kLevelMask) | (kRemovedLevel << kLevelShift); }
+IGPU_HD u32 hdr_revive(u32 h) { return h & ~kLevelMask; }  // back to a live border root
 
 IGPU_HD bool ks_more(u32 ks) { return ks >= kSuffix; }
 IGPU_HD bool ks_record(u32 ks) { return ks == kLongVal || ks == kSuffix; }

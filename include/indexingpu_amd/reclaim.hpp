@@ -88,6 +88,13 @@ class simple_debra_reclaimer {
     IGPU_HIP_TRY(hipMemcpy(&h, view_.leaked, 4, hipMemcpyDeviceToHost));
     return h;
   }
+  // entries in the launch retire list not yet drained (synchronizes)
+  uint32_t pending() const {
+    u32 h = 0;
+    IGPU_HIP_TRY(hipDeviceSynchronize());
+    IGPU_HIP_TRY(hipMemcpy(&h, view_.count, 4, hipMemcpyDeviceToHost));
+    return h;
+  }
 
  private:
   void init(u32 cap) {
@@ -130,3 +137,8 @@ struct tile_retire_cache {
 constexpr u32 kRetireChunk = 16;
 
 }  // namespace indexingpu_amd
+
+// The reference declares its reclaimers in the global namespace
+// (include/simple_debra_reclaim.hpp:32-33, include/simple_dummy_reclaim.hpp:26-27).
+using indexingpu_amd::simple_debra_reclaimer;
+using indexingpu_amd::simple_dummy_reclaimer;

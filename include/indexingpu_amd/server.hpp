@@ -13,8 +13,10 @@
 //     on the device and requests that arrive while others run join the next wave;
 //   * completes a wave's requests when its event fires (latency = completion time -
 //     arrival time) and frees its scan rows.
-// With coalesce = false it runs the plain path instead: one request per launch pair
-// through the index's own mixed_batch / scan calls on one stream, in arrival order.
+// With coalesce = false it runs the plain path instead: the index's own mixed_batch /
+// scan calls on one stream, in arrival order, one call pair at a time -- one request
+// per pair, or (plain_batch) every request that arrived while the previous pair ran
+// (interval batching, the natural way to serve with the plain calls).
 //
 // Ordered waves (ordered = 1).  The point ops of the waves in flight are
 // queued on one HIP stream instead of running side by side, so no wave's writes run
@@ -62,6 +64,7 @@ struct serve_config {
   uint32_t epoch_waves = 32;         // waves between retire-list flips
   uint32_t coalesce = 1;             // 0: plain path (one request per launch pair)
   uint32_t ordered = 0;              // 1: point ops of all waves on one stream, read-only passes
+  uint32_t plain_batch = 0;          // plain path: 1 = one call pair over all arrived requests
 };
 
 // Device arena of a trace.  Point ops: mixed-batch layout (types, keys, key_lengths,
@@ -254,8 +257,9 @@ class masstree_server {
         uint64_t last = admitted, ops = 0, scans = 0, writes = 0;
         while (last < arrived) {
           const serve_request& q = req[last];
-          if (last > admitted && (!cfg_.coalesce || ops + q.num_ops > cfg_.max_wave_ops ||
-                                  scans + q.num_scans > cfg_.max_wave_scans))
+          if (last > admitted &&
+              ((!cfg_.coalesce && !cfg_.plain_batch) || ops + q.num_ops > cfg_.max_wave_ops ||
+               scans + q.num_scans > cfg_.max_wave_scans))
             break;
           // the plain path has no admission control (it is the index's own calls)
           const uint64_t need = (writes + q.num_writes) * (uint64_t)cfg_.slabs_per_write;

@@ -315,3 +315,10 @@ __device__ __forceinline__ void free_retired(const slab_pool_view& v, u32 r) {
 
 }  // namespace dev
 }  // namespace indexingpu_amd
+
+// The reference declares its allocators in the global namespace
+// (include/simple_slab_alloc.hpp:28-29, include/simple_slab_linear_alloc.hpp:31-32,
+// include/simple_bump_alloc.hpp); callers spell them that way.
+using indexingpu_amd::simple_bump_allocator;
+using indexingpu_amd::simple_slab_allocator;
+using indexingpu_amd::simple_slab_linear_allocator;

@@ -59,8 +59,11 @@ struct wave_tile {
   tile_slab_cache ac;
   tile_retire_cache rc;
 
-  __device__ __forceinline__ wave_tile(const slab_pool_view& p, const retire_view& r, u32* e)
-      : pool(p), rv(r), err(e) {
+  u32 tree_root;  // root of the top layer (a write restarts there after a removed layer)
+
+  __device__ __forceinline__ wave_tile(const slab_pool_view& p, const retire_view& r, u32* e,
+                                       u32 root = kInvalid)
+      : pool(p), rv(r), err(e), tree_root(root) {
     const u32 l64 = __lane_id();
     lane = l64 & (T - 1);
     slot = lane & 15u;

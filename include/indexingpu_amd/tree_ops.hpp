@@ -28,9 +28,12 @@
 //     place, so a layer's root index never changes.
 //   * A key tail that collides with an existing suffix record is pushed one layer
 //     down (a new layer root holding the old entry) and the insert continues there.
-//   * Erase removes the entry without merging nodes (no node is ever freed, so an
-//     optimistic reader can never follow a pointer into a recycled node); records are
-//     retired to the launch's reclaimer and freed after the launch.
+//   * Erase removes the entry without merging nodes.  A layer root below the top layer
+//     that an erase leaves empty is removed: marked removed while locked, its LINK taken
+//     out of the layer above, then unlocked and retired (cascading up while layer roots
+//     empty).  Writers that meet a removed node restart from the tree root; readers see
+//     an empty node.  Records and removed nodes are retired to the reclaimer and freed
+//     once no reader of the launch can hold them.
 #pragma once
 #include "layout.hpp"
 
@@ -53,6 +56,9 @@ enum op_phase : u32 {
   ph_parent_descend = 3,  // split: find the level that must receive (sep -> left, right)
   ph_parent_lock = 4,     // split: lock that parent and insert the separator
   ph_done = 5,
+  ph_unlink_descend = 6,  // erase emptied layer root a0: walk from the tree root to the
+                          // border node of layer a1 that holds its LINK
+  ph_unlink_lock = 7,     // lock that node and remove the LINK
 };
 
 // One request, uniform over the tile.
@@ -64,10 +70,13 @@ struct request {
   u32 value_length;
   u32* out_value;       // find output (max_value_length slices)
   u32 max_value_length;
-  // insert / upsert with enable_suffix = false: a new key's slices past the current
-  // layer become one single-entry layer each instead of a suffix record
-  u32 no_suffix;
+  u32 flags;  // kReqNoSuffix | kReqKeepEmptyLayers
 };
+// insert / upsert with enable_suffix = false: a new key's slices past the current layer
+// become one single-entry layer each instead of a suffix record
+constexpr u32 kReqNoSuffix = 1u;
+// erase<do_remove_empty_root = false>: a layer emptied by an erase stays in place
+constexpr u32 kReqKeepEmptyLayers = 2u;
 
 template <class NodeImage>
 struct op_state {
@@ -96,11 +105,48 @@ constexpr u32 kMaxOpTime = 1u << 21;
 constexpr u32 kMaxStepsPerOp = 1u << 31;
 constexpr u32 kHintLevels = 4;
 
-// true while the operation holds node locks (a split waiting for its parent): such a
-// tile must retry at once rather than back off
+// true while the operation holds node locks (a split waiting for its parent, an emptied
+// layer root waiting to be unlinked): such a tile must retry at once rather than back off
 template <class S>
 IGPU_HD bool op_holds_locks(const S& st) {
+  return st.phase == ph_parent_descend || st.phase == ph_parent_lock ||
+         st.phase == ph_unlink_descend || st.phase == ph_unlink_lock;
+}
+template <class S>
+IGPU_HD bool op_in_split(const S& st) {
   return st.phase == ph_parent_descend || st.phase == ph_parent_lock;
+}
+
+// Restarts a write from the tree root (it met a layer root that an erase removed).
+template <class S>
+IGPU_HD void op_restart(S& st, u32 root) {
+  st.layer = 0;
+  st.node = root;
+  st.layer_root = root;
+  st.snap_hdr = kInvalid;
+  clear_hints(st);
+  st.phase = ph_descend;
+}
+
+// Starts unlinking the emptied layer root st.node (locked, marked removed, written):
+// its LINK lives in layer st.layer - 1; the walk starts at the tree root.
+template <class S>
+IGPU_HD void begin_unlink(S& st, u32 root) {
+  st.a0 = st.node;
+  st.a1 = st.layer - 1;
+  op_restart(st, root);
+  st.phase = ph_unlink_descend;
+}
+
+// Gives a removed layer root back its live header (an unlink that cannot complete):
+// an empty layer stays in place, as when removal is off.
+template <class C>
+IGPU_HD void revive_layer_root(C& c, u32 idx) {
+  typename C::node_t m;
+  c.read_node_locked(idx, m);
+  c.set_header(m, hdr_revive(c.header(m)));
+  c.write_node(idx, m);
+  c.unlock_bump(idx);
 }
 
 template <class S>
@@ -219,6 +265,34 @@ IGPU_HD u32 make_layer_chain(C& c, const request& rq, u32 from) {
   return val;
 }
 
+// true when the border node whose locked header is h is a layer root below the top
+// layer that the removal of one entry leaves empty, and empty layers are removed
+template <class S>
+IGPU_HD bool layer_emptied(const request& rq, const S& st, u32 h) {
+  return hdr_root(h) && hdr_border(h) && hdr_nkeys(h) == 1 && st.layer > 0 &&
+         !(rq.flags & kReqKeepEmptyLayers);
+}
+
+// An erase removed an entry from the locked border node st.node (image n, locked header
+// h before the removal): publish it.  A layer root left empty (below the top layer) is
+// marked removed, stays locked, and its LINK is removed from the layer above
+// (ph_unlink_*) before it is unlocked and retired, as the reference's
+// do_remove_empty_root does (include/gpu_masstree.hpp:916-966).
+template <class C, class S>
+IGPU_HD void erase_publish(C& c, const request& rq, S& st, typename C::node_t& n, u32 h) {
+  if (layer_emptied(rq, st, h)) {
+    c.set_header(n, hdr_remove(hdr_with_nkeys(h, 0)));
+    c.write_node(st.node, n);
+    c.fence_release();
+    st.result = 1;
+    begin_unlink(st, c.tree_root);
+    return;
+  }
+  c.write_node(st.node, n);
+  c.unlock_bump(st.node);
+  op_finish(st, 1);
+}
+
 // Returns true when the step made progress (false: retry later).
 template <bool Validate, class C, class S>
 IGPU_HD bool op_step(C& c, const request& rq, S& st) {
@@ -233,12 +307,14 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
     if (st.t0 == 0) st.t0 = now;
     if (now - st.t0 > kMaxOpTime || st.steps > kMaxStepsPerOp) {
       c.set_error(kErrStepLimit);
-      if (op_holds_locks(st)) {
+      if (op_in_split(st)) {
         // a split waiting for its parent holds both halves (written, not yet
         // published); publish them: the right half is reachable through the left
         // one's sibling link, and a later split of it links it from its left neighbour
         c.unlock_bump(st.a0);
         c.unlock_bump(st.a1);
+      } else if (op_holds_locks(st)) {
+        revive_layer_root(c, st.a0);  // the emptied layer stays, live
       }
       op_finish(st, op_holds_locks(st) ? st.result : 0);
       return true;
@@ -272,6 +348,12 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         return true;
       }
       st.lvl = 0;
+      if (hdr_removed(h) && rq.kind != op_find) {
+        // a layer root an erase removed (reached through a LINK or hint read before):
+        // the key's place is in the layer above now
+        op_restart(st, c.tree_root);
+        return true;
+      }
       if (q > c.high(n) && c.sibling(n) != kInvalid) {
         st.node = c.sibling(n);
         return true;
@@ -367,7 +449,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         // insert a new entry
         u32 nks, nval;
         bool ok;
-        if (more && rq.no_suffix) {
+        if (more && (rq.flags & kReqNoSuffix)) {
           nks = kLink;
           nval = make_layer_chain(c, rq, st.layer + 1);
           ok = nval != kInvalid;
@@ -460,10 +542,8 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
           }
           if (rq.kind == op_erase) {
             c.border_remove(n, pos);
-            c.write_node(st.node, n);
-            c.unlock_bump(st.node);
             c.retire(val);
-            op_finish(st, 1);
+            erase_publish(c, rq, st, n, h);
             return true;
           }
           // update / upsert: same tail, new value
@@ -518,10 +598,8 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       }
       if (rq.kind == op_erase) {
         c.border_remove(n, pos);
-        c.write_node(st.node, n);
-        c.unlock_bump(st.node);
         if (ks == kLongVal) c.retire(val);
-        op_finish(st, 1);
+        erase_publish(c, rq, st, n, h);
         return true;
       }
       {
@@ -657,6 +735,100 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       st.a2 = sep2;
       st.a3 = st.a3 + 1;
       goto_parent(st);
+      return true;
+    }
+
+    case ph_unlink_descend: {
+      // the same B-link descent as ph_descend, through LINK entries down to layer a1
+      node_t& n = st.img;
+      if (!c.template read_node<Validate>(st.node, n, st.lvl != 0)) return false;
+      const u32 h = c.header(n);
+      if (!hdr_border(h)) {
+        u32 child = c.find_child(n, q);
+        u32 next_level = hdr_level(h) - 1;
+        if (child == kInvalid) {
+          child = c.sibling(n);
+          next_level = hdr_level(h);
+        }
+        if (child == kInvalid) {
+          c.set_error(kErrBadIndex);
+          revive_layer_root(c, st.a0);
+          op_finish(st, st.result);
+          return true;
+        }
+        st.node = child;
+        st.lvl = next_level;
+        return true;
+      }
+      st.lvl = 0;
+      if (q > c.high(n) && c.sibling(n) != kInvalid) {
+        st.node = c.sibling(n);
+        return true;
+      }
+      if (st.layer == st.a1) {
+        st.snap_hdr = h;
+        st.fresh = 1;
+        st.phase = ph_unlink_lock;
+        return true;
+      }
+      u32 ks = 0, val = 0;
+      if (c.lookup(n, q, true, ks, val) < 0 || ks != kLink || hdr_removed(h)) {
+        // the path to the emptied layer is gone (not expected: only the holder of a
+        // layer root's lock removes its LINK, and a layer with a LINK is not empty)
+        revive_layer_root(c, st.a0);
+        op_finish(st, st.result);
+        return true;
+      }
+      st.layer += 1;
+      st.node = val;
+      st.layer_root = val;
+      st.snap_hdr = kInvalid;
+      clear_hints(st);
+      return true;
+    }
+
+    case ph_unlink_lock: {
+      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr, st.fresh != 0);
+      st.fresh = 0;
+      if (r == kLockBusy) return false;
+      if (r == kLockStale) {
+        st.phase = ph_unlink_descend;
+        return false;
+      }
+      node_t& n = st.img;
+      const u32 h = st.snap_hdr | kLockBit;
+      c.set_header(n, h);
+      u32 ks = 0, val = 0;
+      const int pos = c.lookup(n, q, true, ks, val);
+      if (pos < 0 || ks != kLink || val != st.a0) {
+        const u32 sib = c.sibling(n);
+        c.unlock_same(st.node);
+        if (q > c.high(n) && sib != kInvalid) {  // the node split meanwhile: move right
+          st.node = sib;
+          st.phase = ph_unlink_descend;
+          return true;
+        }
+        revive_layer_root(c, st.a0);
+        op_finish(st, st.result);
+        return true;
+      }
+      c.border_remove(n, pos);
+      const u32 removed = st.a0;
+      if (layer_emptied(rq, st, h)) {
+        // this layer's root is empty now too: remove it the same way (cascade)
+        c.set_header(n, hdr_remove(hdr_with_nkeys(h, 0)));
+        c.write_node(st.node, n);
+        c.fence_release();
+        c.unlock_bump(removed);
+        c.retire(removed | kRetiredNode);
+        begin_unlink(st, c.tree_root);
+        return true;
+      }
+      c.write_node(st.node, n);
+      c.unlock_bump(st.node);
+      c.unlock_bump(removed);  // stays marked removed: later visitors restart
+      c.retire(removed | kRetiredNode);
+      op_finish(st, st.result);
       return true;
     }
 

@@ -389,6 +389,7 @@ int gmt_server_create(gmt_index* index, const gmt_serve_config* config, gmt_serv
   c.epoch_waves = config->epoch_waves;
   c.coalesce = config->coalesce;
   c.ordered = config->ordered;
+  c.plain_batch = config->plain_batch;
   gmt_server* sv = new gmt_server();
   gmt_impl* p = B(index);
   int rc = guarded([&] {

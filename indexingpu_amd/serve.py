@@ -19,7 +19,7 @@ vp = ctypes.c_void_p
 class ServeConfig(ctypes.Structure):
     _fields_ = [("streams", u32), ("slabs_per_write", u32), ("max_wave_ops", u64),
                 ("max_wave_scans", u64), ("ring_rows", u64), ("epoch_waves", u32),
-                ("coalesce", u32), ("ordered", u32)]
+                ("coalesce", u32), ("ordered", u32), ("plain_batch", u32)]
 
 
 class ServeArena(ctypes.Structure):
@@ -60,11 +60,12 @@ class MasstreeServer:
 
     def __init__(self, tree, streams=4, max_wave_ops=1 << 20, max_wave_scans=1 << 16,
                  ring_rows=1 << 22, slabs_per_write=4, epoch_waves=32, coalesce=True,
-                 ordered=False):
+                 ordered=False, plain_batch=False):
         self._lib = _bind(load_library())
         self.tree = tree
         self.config = ServeConfig(streams, slabs_per_write, max_wave_ops, max_wave_scans,
-                                  ring_rows, epoch_waves, int(bool(coalesce)), int(bool(ordered)))
+                                  ring_rows, epoch_waves, int(bool(coalesce)), int(bool(ordered)),
+                                  int(bool(plain_batch)))
         h = vp()
         self._check(self._lib.gmt_server_create(tree._h, ctypes.byref(self.config), ctypes.byref(h)))
         self._h = h

@@ -58,7 +58,7 @@ request make_request(const batch_args& a, uint64_t i, uint32_t* scratch_out) {
   rq.key = a.keys + i * (uint64_t)a.maxK;
   rq.value = a.values ? a.values + i * (uint64_t)a.maxV : nullptr;
   rq.value_length = a.vlens ? a.vlens[i] : a.maxV;
-  rq.no_suffix = a.no_suffix;
+  rq.flags = a.no_suffix ? kReqNoSuffix : 0u;
   rq.out_value = a.out_values ? a.out_values + i * (uint64_t)a.maxV : scratch_out;
   rq.max_value_length = a.out_values ? a.maxV : 0;
   // a find in a mixed batch writes its value over the request's value slots
@@ -167,6 +167,14 @@ void cm_set_fail(void* h, uint32_t any_every, uint32_t parent_every) {
   M.pool.fail_parent_every = parent_every;
   M.pool.error.store(0);
 }
+// Trie layers reachable from the root.
+uint64_t cm_layers(void* h) {
+  model& M = *static_cast<model*>(h);
+  host_walker w([&](uint32_t idx) -> const uint64_t* {
+    return idx < M.pool.num_slabs ? M.pool.slab(idx) : nullptr;
+  }, nullptr);
+  return w.run(M.root).layers;
+}
 // Non-root nodes that no parent slot names (reachable through sibling links only).
 uint64_t cm_unlinked(void* h) {
   model& M = *static_cast<model*>(h);
@@ -197,7 +205,7 @@ void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32
     std::vector<uint32_t> scratch(ntiles, 0);
     std::vector<uint32_t> active;
     for (uint32_t t = 0; t < ntiles; t++) {
-      ctx.emplace_back(M.pool);
+      ctx.emplace_back(M.pool, M.root);
       cur[t] = t;
       if (t < n) {
         rq[t] = make_request(a, t, &scratch[t]);
@@ -229,7 +237,7 @@ void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32
   std::vector<std::thread> th;
   for (uint32_t t = 0; t < nthreads; t++) {
     th.emplace_back([&, t]() {
-      cpu_tile c(M.pool);
+      cpu_tile c(M.pool, M.root);
       state_t s;
       uint32_t scratch = 0;
       for (uint64_t i = t; i < n; i += nthreads) {

@@ -43,7 +43,8 @@ struct cpu_tile {
   // record head cache (first slab of the record loaded by rec_load)
   u32 rec_idx = kInvalid;
 
-  explicit cpu_tile(shared_pool& p) : P(p) {}
+  u32 tree_root = kInvalid;  // root of the top layer (writes restart there)
+  explicit cpu_tile(shared_pool& p, u32 root = kInvalid) : P(p), tree_root(root) {}
 
   void set_error(u32 code) {
     u32 z = 0;

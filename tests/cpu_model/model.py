@@ -32,6 +32,8 @@ def _load():
         lib.cm_set_no_suffix.argtypes = [vp, u32]
         lib.cm_unlinked.argtypes = [vp]
         lib.cm_unlinked.restype = u64
+        lib.cm_layers.argtypes = [vp]
+        lib.cm_layers.restype = u64
         lib.cm_batch.argtypes = [vp, u8p, u32, u32p, u32, u32p, u32p, u32, u32p, u32p, u32p,
                                  u8p, u64, u32, u32, u64]
         lib.cm_dump.argtypes = [vp, u32p, u32p, u32p, u32p, ctypes.POINTER(u64),
@@ -73,6 +75,16 @@ class CpuModel:
         """Injected allocation failures: every k-th allocation fails, and every k-th
         allocation of a split inserting into its parent (0 = off).  Clears the error."""
         self.lib.cm_set_fail(self.h, any_every, parent_every)
+
+    @property
+    def layers(self):
+        """Trie layers reachable from the root."""
+        return int(self.lib.cm_layers(self.h))
+
+    @property
+    def retired(self):
+        """Objects retired so far (records; nodes carry bit 31)."""
+        return int(self.lib.cm_retired(self.h))
 
     @property
     def unlinked(self):

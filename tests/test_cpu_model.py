@@ -215,3 +215,53 @@ def test_inserts_without_suffix_records(nthreads):
         assert _eq(b.insert(keys[e], lens[e], vals[e], vl[e], mk, mv, upsert=True),
                    o.insert(keys[e], lens[e], vals[e], vl[e], mk, mv, upsert=True))
         assert b.items() == o.items()
+
+
+@pytest.mark.parametrize("no_suffix", [False, True], ids=["suffix_records", "layer_chains"])
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_erase_removes_empty_layers(nthreads, no_suffix):
+    """Erasing every key under a shared prefix empties the layers below it: each emptied
+    layer root is unlinked from the layer above and retired (cascading up through layer
+    chains), while inserts into the same prefixes run in the same batches.  Results equal
+    the oracle's; once every key is gone only the top layer is left.  (Layers here stay
+    single nodes: a layer that grew an interior root keeps its empty nodes.)"""
+    rng = np.random.default_rng(41)
+    n_pre, per = 64, 12
+    pre = rng.integers(0, 1 << 32, size=(n_pre, 3), dtype=np.uint64).astype(np.uint32)
+    keys = np.zeros((n_pre * per, 6), np.uint32)
+    keys[:, :3] = np.repeat(pre, per, axis=0)
+    keys[:, 3:] = rng.integers(0, 1 << 32, size=(n_pre * per, 3), dtype=np.uint64).astype(np.uint32)
+    keys[:, 3] %= 4  # a layer below each prefix, and layers below those
+    lens = rng.integers(4, 7, size=len(keys)).astype(np.uint32)
+    tup = [tuple(keys[i, :lens[i]].tolist()) for i in range(len(keys))]
+    _, first = np.unique(np.array([hash(t) for t in tup]), return_index=True)
+    keys, lens = keys[np.sort(first)], lens[np.sort(first)]
+    pid = np.array([np.nonzero((pre == k[:3]).all(1))[0][0] for k in keys])
+    vals = rng.integers(0, 1 << 32, size=(len(lens), 2), dtype=np.uint64).astype(np.uint32)
+    mk, mv = 6, 2
+    b = S.CpuModelBackend(nthreads=nthreads, no_suffix=no_suffix)
+    o = S.OracleBackend()
+    _eq = np.array_equal
+    assert _eq(b.insert(keys, lens, vals, None, mk, mv), o.insert(keys, lens, vals, None, mk, mv))
+    full = b.m.layers
+    gone = np.isin(pid, np.arange(0, n_pre, 2))  # erase every key of the even prefixes
+    e = np.nonzero(gone)[0]
+    assert _eq(b.erase(keys[e], lens[e], mk), o.erase(keys[e], lens[e], mk))
+    assert b.items() == o.items()
+    assert b.m.layers < full
+    # erase the rest while re-inserting the even prefixes' keys, as one mixed batch
+    k = np.nonzero(~gone)[0]
+    types = np.concatenate([np.ones(len(k), np.uint8), np.zeros(len(e), np.uint8)])
+    idx = np.concatenate([k, e])
+    p = rng.permutation(len(idx))
+    types, idx = types[p], idx[p]
+    vl = np.full(len(idx), mv, np.uint32)
+    for g, x in zip(b.mixed(types, keys[idx], lens[idx], vals[idx], vl, mk, mv),
+                    o.mixed(types, keys[idx], lens[idx], vals[idx], vl, mk, mv)):
+        assert _eq(g, x)
+    assert b.items() == o.items()
+    for g, x in zip(b.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        assert _eq(g, x)
+    # everything erased: only the top layer is left
+    assert _eq(b.erase(keys, lens, mk), o.erase(keys, lens, mk))
+    assert b.items() == [] and b.m.layers == 1

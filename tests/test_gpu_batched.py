@@ -158,3 +158,76 @@ def test_batched_form_captured_into_a_graph():
         assert np.array_equal(got[i], np.concatenate([e[i] for e in exp]))
     assert g.items() == o.items()
     del graph
+
+
+def _prefix_batches(n_keys, n_batches, seed):
+    from indexingpu_amd import device_workloads as D
+    wl = D.PrefixBatchWorkload(n_keys, seed=seed)
+    rng = np.random.default_rng(seed)
+    sizes = D.log_uniform_sizes(rng, n_batches, 64, 8192)
+    batches = [wl.batch(int(s)) for s in sizes]
+    return wl, batches, D.concat_batches(batches)
+
+
+def _h32(t):
+    return t.cpu().numpy().view(np.uint32) if t.dtype == torch.int32 else t.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode,split,captured", [(0, None, False), (1, 0xFFFFFFFF, False),
+                                                 (1, 0, False), (1, 0, True),
+                                                 (1, 0xFFFFFFFF, True)])
+def test_prefixed_16_slice_batches(mode, split, captured):
+    """The batched form's own workload shape: 16-slice keys whose first 12 slices are one
+    of 256 shared prefixes (deep trie layers, suffix records), ~50K preloaded, uneven
+    batches of 90% Zipf finds / 5% fresh inserts / 5% erases, applied by one multi_batch
+    call in both modes, eagerly and captured into a graph.  Every output equals the oracle
+    replaying the batches one after another and the generator's own expectation."""
+    wl, batches, cat = _prefix_batches(50_000, 60, seed=13 + mode)
+    mk, mv = wl.key_len, wl.mv
+    pk, pl, pv = wl.preload()
+    g, o = S.GpuBackend(16, pool_bytes=256 << 20), S.OracleBackend()
+    g.t.set_multi_mode(mode)
+    if split is not None:
+        g.t.set_mixed_split(split)
+    keys_np, lens_np, vals_np = _h32(pk), _h32(pl), _h32(pv)
+    n0 = len(lens_np)
+    r = g._zeros(n0, np.uint8)
+    g.t.insert(pk.view(-1), mk, pl, pv.reshape(-1), mv, None, n0, results=r)
+    o.insert(keys_np.reshape(n0, mk), lens_np, vals_np.reshape(n0, mv), None, mk, mv)
+    total = len(cat["types"])
+    v = cat["values"].reshape(-1).clone()
+    vl = torch.full((total,), mv, dtype=torch.int32, device="cuda")
+    res = torch.zeros(total, dtype=torch.uint8, device="cuda")
+    nb = len(batches)
+
+    def call():
+        g.t.multi_batch(cat["types"], cat["keys"].reshape(-1), mk, cat["lens"], v, mv, vl, res,
+                        cat["offsets"], nb, total)
+    if captured:
+        g.t.reserve_find_scratch(total)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            call()
+        graph.replay()
+    else:
+        call()
+    torch.cuda.synchronize()
+    g.t.check_device_error()
+    gv, gl, gr = _h32(v).reshape(total, mv), _h32(vl), res.cpu().numpy()
+    # the generator's expectation
+    assert np.array_equal(gl, _h32(cat["expect_len"]))
+    assert np.array_equal(gv, _h32(cat["expect_values"]).reshape(total, mv))
+    assert np.array_equal(gr, cat["expect_results"].cpu().numpy())
+    # the oracle replaying the batches in order
+    exp = []
+    for b in batches:
+        n = len(b["types"])
+        exp.append(o.mixed(b["types"].cpu().numpy(), _h32(b["keys"]).reshape(n, mk),
+                           _h32(b["lens"]), _h32(b["values"]).reshape(n, mv),
+                           np.full(n, mv, np.uint32), mk, mv))
+    for i, got in enumerate((gv, gl, gr)):
+        assert np.array_equal(got, np.concatenate([e[i] for e in exp]))
+    assert g.items() == o.items()
+    if captured:
+        del graph

@@ -1,0 +1,302 @@
+"""A caller written in the reference's own spelling of the API (tests/refspell: the
+reference's include names and global allocator / reclaimer / context names,
+GpuMasstree::gpu_masstree at tiles 16 and 32 over simple_slab_allocator and
+simple_slab_linear_allocator, insert<..., enable_suffix = false>, the erase merge-level
+dispatch of evaluation/gpu_masstree_adapter.hpp:215-237, scan<false>, mixed_batch, and
+user kernels calling cooperative_find / _insert / _erase / _scan with HIP
+cooperative-groups tiles and the per-block reclaimer context), compiled against this
+repository's include/ only.  Every result equals the oracle's."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import scenarios as S
+from indexingpu_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+LIB = os.path.join(os.path.dirname(__file__), "refspell", "librefspell_probe.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is missing: run __graft_entry__.build()")
+        l = ctypes.CDLL(LIB)
+        vp, u32 = ctypes.c_void_p, ctypes.c_uint32
+        l.rs_create.restype = vp
+        l.rs_create.argtypes = [u32, u32, ctypes.c_float]
+        l.rs_destroy.argtypes = [vp]
+        l.rs_error.restype = u32
+        l.rs_error.argtypes = [vp]
+        l.rs_slabs_used.restype = ctypes.c_uint64
+        l.rs_slabs_used.argtypes = [vp]
+        l.rs_stats.argtypes = [vp]
+        l.rs_insert.argtypes = [vp, vp, u32, vp, vp, u32, vp, u32, ctypes.c_int, ctypes.c_int]
+        l.rs_erase.argtypes = [vp, vp, u32, vp, u32, u32]
+        l.rs_find.argtypes = [vp, vp, u32, vp, vp, u32, vp, u32, ctypes.c_int]
+        l.rs_scan.argtypes = [vp, vp, vp, u32, u32, u32, vp, vp, vp, vp, u32, vp, vp, vp]
+        l.rs_mixed.argtypes = [vp, vp, vp, u32, vp, vp, u32, vp, vp, u32, ctypes.c_int]
+        l.rs_user_point.argtypes = [vp, u32, vp, vp, u32, vp, vp, u32, vp, vp, vp, vp, u32, u32,
+                                    ctypes.c_int]
+        l.rs_pending_retired.restype = u32
+        l.rs_pending_retired.argtypes = [vp]
+        l.rs_drain.argtypes = [vp]
+        l.rs_user_scan.argtypes = [vp, vp, vp, vp, vp, u32, u32, vp, vp, u32, vp, vp, vp, u32]
+        _lib = l
+    return _lib
+
+
+def _d(a, dtype=np.uint32):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+    view = np.int32 if dtype == np.uint32 else np.uint8
+    return torch.from_numpy(a.view(view).copy()).cuda()
+
+
+def _z(n, dtype=torch.int32):
+    return torch.zeros(max(n, 1), dtype=dtype, device="cuda")
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _h(t, n=None):
+    torch.cuda.synchronize()
+    a = t.cpu().numpy()
+    a = a.view(np.uint32) if a.dtype == np.int32 else a
+    return a if n is None else a[:n]
+
+
+class RefSpelled:
+    def __init__(self, tile, linear):
+        self.l = lib()
+        self.h = self.l.rs_create(tile, linear, 0.004)
+        assert self.h
+
+    def check(self):
+        torch.cuda.synchronize()
+        assert self.l.rs_error(self.h) == 0
+
+    def insert(self, keys, lens, vals, vl, mk, mv, upsert=False, enable_suffix=True):
+        n = len(lens)
+        k, kl, v, l = _d(keys), _d(lens), _d(vals), _d(vl)  # held until the call is done
+        self.l.rs_insert(self.h, _p(k), mk, _p(kl), _p(v), mv, _p(l), n, int(upsert),
+                         int(enable_suffix))
+        self.check()
+
+    def erase(self, keys, lens, mk, merge_level):
+        k, kl = _d(keys), _d(lens)
+        self.l.rs_erase(self.h, _p(k), mk, _p(kl), len(lens), merge_level)
+        self.check()
+
+    def find(self, keys, lens, mk, mv, concurrent=False):
+        n = len(lens)
+        k, kl = _d(keys), _d(lens)
+        v, vl = _z(n * mv), _z(n)
+        self.l.rs_find(self.h, _p(k), mk, _p(kl), _p(v), mv, _p(vl), n, int(concurrent))
+        self.check()
+        return _h(v, n * mv).reshape(n, mv), _h(vl, n)
+
+    def _scan_out(self, nq, maxc, mk, mv):
+        return _z(nq), _z(nq * maxc * mv), _z(nq * maxc), _z(nq * maxc * mk), _z(nq * maxc)
+
+    def _scan_res(self, outs, nq, maxc, mk, mv):
+        c, v, vl, k, kl = outs
+        return (_h(c, nq), _h(v, nq * maxc * mv), _h(vl, nq * maxc), _h(k, nq * maxc * mk),
+                _h(kl, nq * maxc))
+
+    def scan(self, lo, lol, mk, maxc, mv, up=None, upl=None):
+        nq = len(lol)
+        a, al, b, bl = _d(lo), _d(lol), _d(up), _d(upl)
+        outs = self._scan_out(nq, maxc, mk, mv)
+        c, v, vl, k, kl = outs
+        self.l.rs_scan(self.h, _p(a), _p(al), mk, maxc, nq, _p(b), _p(bl), _p(c), _p(v), mv,
+                       _p(vl), _p(k), _p(kl))
+        self.check()
+        return self._scan_res(outs, nq, maxc, mk, mv)
+
+    def user_scan(self, lo, lol, mk, maxc, mv, up=None, upl=None):
+        nq = len(lol)
+        a, al, b, bl = _d(lo), _d(lol), _d(up), _d(upl)
+        outs = self._scan_out(nq, maxc, mk, mv)
+        c, v, vl, k, kl = outs
+        self.l.rs_user_scan(self.h, _p(a), _p(al), _p(b), _p(bl), mk, maxc, _p(c), _p(v), mv,
+                            _p(vl), _p(k), _p(kl), nq)
+        self.check()
+        return self._scan_res(outs, nq, maxc, mk, mv)
+
+    def mixed(self, types, keys, lens, vals, vl, mk, mv, merge_level=3):
+        n = len(types)
+        t, k, kl, v, l = _d(types, np.uint8), _d(keys), _d(lens), _d(vals), _d(vl)
+        r = _z(n, torch.uint8)
+        self.l.rs_mixed(self.h, _p(t), _p(k), mk, _p(kl), _p(v), mv, _p(l), _p(r), n,
+                        merge_level)
+        self.check()
+        return _h(v, n * mv).reshape(n, mv), _h(l, n), _h(r, n)
+
+    def user_point(self, op, keys, lens, mk, vals=None, vl=None, mv=1, blocks=256, types=None,
+                   drain=True):
+        n = len(lens)
+        t, k, kl, v, l = _d(types, np.uint8), _d(keys), _d(lens), _d(vals), _d(vl)
+        out, outl, res = _z(n * mv), _z(n), _z(n, torch.uint8)
+        self.l.rs_user_point(self.h, op, _p(t), _p(k), mk, _p(kl), _p(v), mv, _p(l), _p(out),
+                             _p(outl), _p(res), n, blocks, int(drain))
+        self.check()
+        return _h(out, n * mv).reshape(n, mv), _h(outl, n), _h(res, n)
+
+    def slabs(self):
+        return int(self.l.rs_slabs_used(self.h))
+
+    def close(self):
+        self.l.rs_destroy(self.h)
+
+
+def _keys(seed, n, mk, shared=32):
+    rng = np.random.default_rng(seed)
+    keys, lens = W.unique_random_keys(rng, n, 1, mk)
+    keys[:, 0] %= shared  # layers and suffix records
+    _, first = np.unique(W.key_hash64(keys, lens), return_index=True)
+    first = np.sort(first)
+    return keys[first], lens[first], rng
+
+
+@pytest.mark.parametrize("linear", [0, 1], ids=["slab_alloc", "slab_linear_alloc"])
+@pytest.mark.parametrize("tile", [16, 32])
+def test_reference_spelled_host_calls(tile, linear):
+    mk, mv = 6, 3
+    keys, lens, rng = _keys(7 + tile + linear, 40000, mk)
+    n = len(lens)
+    vl = rng.integers(0, mv + 1, size=n).astype(np.uint32)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    g, o = RefSpelled(tile, linear), S.OracleBackend()
+    try:
+        third = n // 3
+        # a third with suffix records, a third as layer chains (enable_suffix = false),
+        # the last third upserted over both
+        g.insert(keys[:third], lens[:third], vals[:third], vl[:third], mk, mv)
+        o.insert(keys[:third], lens[:third], vals[:third], vl[:third], mk, mv)
+        g.insert(keys[third:2 * third], lens[third:2 * third], vals[third:2 * third],
+                 vl[third:2 * third], mk, mv, enable_suffix=False)
+        o.insert(keys[third:2 * third], lens[third:2 * third], vals[third:2 * third],
+                 vl[third:2 * third], mk, mv)
+        g.insert(keys, lens, vals ^ np.uint32(7), vl, mk, mv, upsert=True, enable_suffix=False)
+        o.insert(keys, lens, vals ^ np.uint32(7), vl, mk, mv, upsert=True)
+        for conc in (False, True):
+            for a, b in zip(g.find(keys, lens, mk, mv, conc), o.find(keys, lens, mk, mv)):
+                assert np.array_equal(a, b)
+        # erase a quarter at each merge level of the adapter's dispatch
+        perm = rng.permutation(n)
+        for lvl in range(4):
+            e = perm[lvl * (n // 8):(lvl + 1) * (n // 8)]
+            g.erase(keys[e], lens[e], mk, lvl)
+            o.erase(keys[e], lens[e], mk)
+        q = rng.choice(n, 2000, replace=False)
+        for a, b in zip(g.scan(keys[q], lens[q], mk, 20, mv), o.scan(keys[q], lens[q], mk, 20, maxV=mv)):
+            assert np.array_equal(a, b)
+        for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+            assert np.array_equal(a, b)
+        # a mixed batch: finds of untouched keys, inserts of erased keys, erases of others
+        rest = perm[n // 2:]
+        m = 6000
+        types = np.full(m, 2, np.uint8)
+        idx = np.empty(m, np.int64)
+        idx[:4000] = rest[:4000]
+        types[4000:5000] = 0
+        idx[4000:5000] = perm[:1000]
+        types[5000:] = 1
+        idx[5000:] = rest[4000:5000]
+        mvals = vals[idx].copy()
+        mvl = np.full(m, mv, np.uint32)
+        for a, b in zip(g.mixed(types, keys[idx], lens[idx], mvals, mvl, mk, mv),
+                        o.mixed(types, keys[idx], lens[idx], mvals, mvl, mk, mv)):
+            assert np.array_equal(a, b)
+        g.l.rs_stats(g.h)
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("tile", [16, 32])
+def test_reference_spelled_user_kernels(tile):
+    """User kernels in the shape of the reference's batch kernel: cooperative_insert /
+    _find / _erase with the per-block reclaimer context (begin / end critical section
+    each round, drain_all at exit) and cooperative_scan with and without upper keys."""
+    mk, mv = 8, 4
+    keys, lens, rng = _keys(70 + tile, 30000, mk, shared=16)
+    n = len(lens)
+    vl = rng.choice([0, 1, 2, 4], size=n).astype(np.uint32)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    g, o = RefSpelled(tile, 0), S.OracleBackend()
+    try:
+        _, _, r = g.user_point(0, keys, lens, mk, vals, vl, mv)
+        assert np.array_equal(r, o.insert(keys, lens, vals, vl, mk, mv))
+        v, l, _ = g.user_point(2, keys, lens, mk, mv=mv)
+        ev, el = o.find(keys, lens, mk, mv)
+        assert np.array_equal(l, el) and np.array_equal(v, ev)
+        e = rng.choice(n, n // 3, replace=False)
+        _, _, r = g.user_point(1, keys[e], lens[e], mk)
+        assert np.array_equal(r, o.erase(keys[e], lens[e], mk))
+        q = rng.choice(n, 3000, replace=False)
+        got = g.user_scan(keys[q], lens[q], mk, 30, mv)
+        exp = o.scan(keys[q], lens[q], mk, 30, maxV=mv)
+        for a, b in zip(got, exp):
+            assert np.array_equal(a, b)
+        # upper bounds: each query's upper key is another stored key
+        hi = rng.choice(n, 3000)
+        lo_k, hi_k = keys[q].copy(), keys[hi].copy()
+        lo_l, hi_l = lens[q].copy(), lens[hi].copy()
+        got = g.user_scan(lo_k, lo_l, mk, 30, mv, hi_k, hi_l)
+        exp = o.scan(lo_k, lo_l, mk, 30, hi_k, hi_l, maxV=mv)
+        for a, b in zip(got, exp):
+            assert np.array_equal(a, b)
+    finally:
+        g.close()
+
+
+def test_in_kernel_epochs_reuse_slabs():
+    """One user-kernel launch erases the resident half of the keys and inserts the other
+    half, interleaved, with the per-block reclaimer context: most erased records are freed
+    inside the launch once the epoch has moved on three times (few blocks, many rounds
+    each), so drain_all hands only the last few epochs' to the host; results equal the
+    oracle's, and the pool stays bounded over rounds."""
+    mk, mv = 4, 6  # every entry has a record (suffix or long value)
+    keys, lens, rng = _keys(99, 40000, mk, shared=1 << 30)
+    n = len(lens)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    vl = np.full(n, mv, np.uint32)
+    g, o = RefSpelled(16, 0), S.OracleBackend()
+    try:
+        half = n // 2
+        _, _, r = g.user_point(0, keys[:half], lens[:half], mk, vals[:half], vl[:half], mv)
+        assert np.array_equal(r, o.insert(keys[:half], lens[:half], vals[:half], vl[:half], mk, mv))
+        base = g.slabs()
+        for rnd in range(4):
+            a = np.arange(0, half) if rnd % 2 == 0 else np.arange(half, 2 * half)
+            b = np.arange(half, 2 * half) if rnd % 2 == 0 else np.arange(0, half)
+            idx = np.empty(2 * half, np.int64)
+            idx[0::2], idx[1::2] = a, b
+            types = np.empty(2 * half, np.uint8)
+            types[0::2], types[1::2] = 1, 0
+            assert g.l.rs_pending_retired(g.h) == 0
+            _, _, r = g.user_point(0, keys[idx], lens[idx], mk, vals[idx], vl[idx], mv, blocks=8,
+                                   types=types, drain=False)
+            exp = np.zeros(2 * half, np.uint8)
+            exp[0::2] = o.erase(keys[a], lens[a], mk)
+            exp[1::2] = o.insert(keys[b], lens[b], vals[b], vl[b], mk, mv)
+            assert np.array_equal(r, exp)
+            handed = g.l.rs_pending_retired(g.h)
+            assert handed < half // 4, (handed, half)  # the rest was freed in the launch
+            g.l.rs_drain(g.h)
+        assert g.slabs() < base * 1.3
+        v, l, _ = g.user_point(2, keys, lens, mk, mv=mv)
+        ev, el = o.find(keys, lens, mk, mv)
+        assert np.array_equal(l, el) and np.array_equal(v, ev)
+    finally:
+        g.close()
