@@ -133,6 +133,68 @@ def cpu_baseline(n_keys, seed, threads, oracle_sample):
                 insert_mops=oracle_sample / (t4 - t3) / 1e6, mt_same=same)
 
 
+def run_cpu_path(torch, args, threads):
+    """The reference's CPU-path workload (1,000,000 unique 2-slice keys of fixed length,
+    key_lengths = nullptr; one-slice values = FNV-1a-32 of the key; then 1,000,000 finds,
+    50% stored and 50% fresh keys) on the GPU index and on the CPU index in the reference
+    CPU adapter's per-key shape (insert overwrites, absent = 0xFFFFFFFF, big-endian byte
+    keys), each checked against the generator; both print the reference bench's result
+    lines ("<op>: <avg> Mop/s (<runs>; ...)")."""
+    from indexingpu_amd import GpuMasstree, workloads as W
+    from oracle.cpu_index import CpuIndex, result_line
+    n = 1_000_000
+    d = W.cpu_path_workload(n, seed=args.seed + 30)
+    keys, vals, fkeys = d["keys"], d["values"].reshape(-1), d["find_keys"]
+    hit = d["find_hit"]
+    hit_exp = W.make_values(fkeys, np.full(n, 2, np.uint32), 1).reshape(-1)
+    tree = GpuMasstree(tile_size=args.tile, pool_bytes=2 << 30)
+    stream = torch.cuda.current_stream()
+    dk, dv, df = up(torch, keys), up(torch, vals), up(torch, fkeys)
+    out_v = torch.zeros(n, dtype=torch.int32, device="cuda")
+    out_l = torch.zeros(n, dtype=torch.int32, device="cuda")
+
+    def ev_ms(fn):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        s.record(stream)
+        fn()
+        e.record(stream)
+        torch.cuda.synchronize()
+        return s.elapsed_time(e)
+    t_ins = ev_ms(lambda: tree.insert(dk, 2, None, dv, 1, None, n))
+    runs = [n / ev_ms(lambda: tree.find(df, 2, None, out_v, 1, out_l, n)) / 1e3 for _ in range(5)]
+    tree.check_device_error()
+    gl = out_l.cpu().numpy().view(np.uint32)
+    gv = out_v.cpu().numpy().view(np.uint32)
+    gpu_ok = bool((gl[hit] == 1).all() and (gl[~hit] == 0).all()
+                  and (gv[hit] == hit_exp[hit]).all())
+    tree.close()
+    del dk, dv, df, out_v, out_l
+    c = CpuIndex(shard_bits=10, threads=threads)
+    t0 = time.perf_counter()
+    c.put1(keys.reshape(-1), 2, None, vals, n)
+    t1 = time.perf_counter()
+    cruns, got = [], None
+    for _ in range(3):
+        t2 = time.perf_counter()
+        got = c.get1(fkeys.reshape(-1), 2, None, n)
+        cruns.append(n / (time.perf_counter() - t2) / 1e6)
+    cpu_ok = bool((got[hit] == hit_exp[hit]).all() and (got[~hit] == 0xFFFFFFFF).all()
+                  and int((got != 0xFFFFFFFF).sum()) == n // 2)
+    lines = {"gpu": [result_line("insert", [round(n / t_ins / 1e3, 3)]),
+                     result_line("find", [round(r, 3) for r in runs])],
+             "cpu": [result_line("insert", [round(n / (t1 - t0) / 1e6, 3)]),
+                     result_line("find", [round(r, 3) for r in cruns])]}
+    for side in ("gpu", "cpu"):
+        for ln in lines[side]:
+            log(f"[cpu path] {side} {ln}")
+    return dict(cpu_path=dict(keys=n, key_slices=2, value_slices=1, gpu_insert_mops=round(n / t_ins / 1e3, 2),
+                              gpu_find_mops=round(float(np.mean(runs)), 2), gpu_verified=gpu_ok,
+                              cpu_insert_mops=round(n / (t1 - t0) / 1e6, 3),
+                              cpu_find_mops=round(float(np.mean(cruns)), 3), cpu_threads=threads,
+                              cpu_verified=cpu_ok, result_lines=lines))
+
+
 def run_mixed(torch, args):
     """Mixed batches on the plain path: n preloaded 4-slice keys, 1M-op batches of Zipf
     (0.99) finds / fresh inserts / erases of keys not read in the batch (90/5/5), then
@@ -720,6 +782,7 @@ def main():
 
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
+        result.update(run_cpu_path(torch, args, threads))
         log(f"[bench] CPU baselines: {n} keys ({threads} threads), oracle on "
             f"{args.cpu_sample} keys ...")
         cb = cpu_baseline(n, args.seed, threads, args.cpu_sample)

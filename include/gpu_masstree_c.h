@@ -10,7 +10,13 @@
  * The reference exposes this path as the C++ template API
  * GpuMasstree::gpu_masstree<Allocator, Reclaimer, tile_size> (include/gpu_masstree.hpp
  * in hyoungjook/IndexinGPU); each entry point below names the member it stands for.
- * The same API is available here as C++ in <indexingpu_amd/gpu_masstree.hpp>.
+ * The same API is available here as C++ in <indexingpu_amd/gpu_masstree.hpp> (and under
+ * the reference's include names, <gpu_masstree.hpp> etc.).
+ *
+ * Concurrency: every call on one index, read-only ones included, must be ordered on one
+ * stream.  Mutating calls share the index's reclaimer, and finds, mixed batches and
+ * hinted writes share its scratch (per-request words indexed from each batch's first
+ * request); two finds on different streams would overwrite each other's scratch.
  */
 #ifndef GPU_MASSTREE_C_H
 #define GPU_MASSTREE_C_H

@@ -176,7 +176,7 @@ def cpu_path_workload(n_keys, seed=1):
     fk = np.concatenate([keys[rng.integers(0, n_keys, size=hit)], fresh_k])
     perm = rng.permutation(n_keys)
     return dict(keys=keys, values=vals, find_keys=np.ascontiguousarray(fk[perm]),
-                max_key_length=2, max_value_length=1)
+                find_hit=perm < hit, max_key_length=2, max_value_length=1)
 
 
 class MixedWorkload:

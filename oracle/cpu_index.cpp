@@ -11,6 +11,15 @@
 // keeps the global key order, so a scan walks the shards in order), each behind a
 // reader/writer lock.  Key order and value semantics are the oracle's (lexicographic
 // u32 slices, proper prefix first; value lengths with 0 = absent on find).
+//
+// Conventions of the reference's CPU adapter kept here:
+//   * keys are stored as byte strings, each u32 slice byte-swapped to big-endian, so that
+//     byte order equals slice order (include/generate_workload.hpp:291-297 and
+//     evaluation/universal_bench.cu:136-140 swap the slices before handing them to
+//     masstree-beta); a proper prefix sorts first, as in the GPU index;
+//   * the adapter-shaped entry points (cpx_put1 / cpx_get1 / cpx_scan) take one-slice
+//     values, report an absent key as 0xFFFFFFFF (cpu_masstree_adapter.hpp:42,120-125)
+//     and insert by overwriting an existing value (cpu_masstree_adapter.hpp:96-104).
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -18,13 +27,15 @@
 #include <memory>
 #include <mutex>
 #include <shared_mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
 namespace {
 
-using Key = std::vector<uint32_t>;
+using Key = std::string;  // big-endian bytes of the u32 slices
 using Val = std::vector<uint32_t>;
+constexpr uint32_t kAbsent = 0xFFFFFFFFu;
 
 struct shard {
   std::shared_mutex mu;
@@ -37,13 +48,26 @@ struct cpu_index {
   explicit cpu_index(uint32_t b) : bits(b) {
     for (uint32_t i = 0; i < (1u << b); i++) shards.emplace_back(new shard());
   }
-  shard& of(const Key& k) { return *shards[bits ? (k.empty() ? 0 : (k[0] >> (32 - bits))) : 0]; }
+  // shard of a key: the top `bits` bits of its first slice (its first bytes)
+  uint32_t shard_of(const Key& k) const {
+    if (!bits || k.empty()) return 0;
+    uint32_t first = 0;
+    for (size_t b = 0; b < 4; b++)
+      first = (first << 8) | (b < k.size() ? (uint8_t)k[b] : 0u);
+    return first >> (32 - bits);
+  }
+  shard& of(const Key& k) { return *shards[shard_of(k)]; }
 };
 
 Key make_key(const uint32_t* keys, uint32_t maxK, const uint32_t* lens, uint64_t i) {
   const uint32_t L = lens ? lens[i] : maxK;
   const uint32_t* p = keys + i * (uint64_t)maxK;
-  return Key(p, p + L);
+  Key k(4ull * L, '\0');
+  for (uint32_t j = 0; j < L; j++) {
+    const uint32_t v = __builtin_bswap32(p[j]);
+    std::memcpy(&k[4ull * j], &v, 4);
+  }
+  return k;
 }
 
 // runs f(i) for i in [0, n) on `threads` workers, worker t taking i = t, t + threads, ...
@@ -139,15 +163,42 @@ void cpx_scan(void* h, const uint32_t* lower, uint32_t maxK, const uint32_t* len
     Key k = make_key(lower, maxK, lens, q);
     uint32_t* row = out_values + q * (uint64_t)count;
     uint32_t c = 0;
-    const uint32_t first = x->bits ? (k.empty() ? 0 : (k[0] >> (32 - x->bits))) : 0;
+    const uint32_t first = x->shard_of(k);
     for (uint32_t sh = first; sh < x->shards.size() && c < count; sh++) {
       shard& s = *x->shards[sh];
       std::shared_lock<std::shared_mutex> g(s.mu);
       for (auto it = s.m.lower_bound(k); it != s.m.end() && c < count; ++it)
-        row[c++] = it->second.empty() ? 0xFFFFFFFFu : it->second[0];
+        row[c++] = it->second.empty() ? kAbsent : it->second[0];
     }
     if (counts) counts[q] = c;
-    for (uint32_t j = c; j < count; j++) row[j] = 0xFFFFFFFFu;
+    for (uint32_t j = c; j < count; j++) row[j] = kAbsent;
+  });
+}
+
+// The adapter's insert (cpu_masstree_adapter.hpp:96-104): one-slice value, an existing
+// key's value is overwritten.
+void cpx_put1(void* h, const uint32_t* keys, uint32_t maxK, const uint32_t* lens,
+              const uint32_t* values, uint64_t n, uint32_t threads) {
+  auto* x = static_cast<cpu_index*>(h);
+  striped(n, threads, [&](uint64_t i) {
+    Key k = make_key(keys, maxK, lens, i);
+    shard& s = x->of(k);
+    std::unique_lock<std::shared_mutex> g(s.mu);
+    s.m[std::move(k)].assign(1, values[i]);
+  });
+}
+
+// The adapter's find (cpu_masstree_adapter.hpp:120-125): the first value slice, or
+// 0xFFFFFFFF for an absent key.
+void cpx_get1(void* h, const uint32_t* keys, uint32_t maxK, const uint32_t* lens,
+              uint32_t* out, uint64_t n, uint32_t threads) {
+  auto* x = static_cast<cpu_index*>(h);
+  striped(n, threads, [&](uint64_t i) {
+    Key k = make_key(keys, maxK, lens, i);
+    shard& s = x->of(k);
+    std::shared_lock<std::shared_mutex> g(s.mu);
+    auto it = s.m.find(k);
+    out[i] = (it == s.m.end() || it->second.empty()) ? kAbsent : it->second[0];
   });
 }
 

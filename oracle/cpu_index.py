@@ -28,6 +28,8 @@ def _load():
         lib.cpx_erase.argtypes = [vp, vp, u32, vp, u64, vp, u32]
         lib.cpx_find.argtypes = [vp, vp, u32, vp, vp, u32, vp, u64, u32]
         lib.cpx_scan.argtypes = [vp, vp, u32, vp, u32, u64, vp, vp, u32]
+        lib.cpx_put1.argtypes = [vp, vp, u32, vp, vp, u64, u32]
+        lib.cpx_get1.argtypes = [vp, vp, u32, vp, vp, u64, u32]
         _lib = lib
     return _lib
 
@@ -84,3 +86,26 @@ class CpuIndex:
         self._lib.cpx_scan(self._h, _p(lower), maxK, _p(lens), count, nq, _p(v), _p(c),
                            self.threads)
         return c, v
+
+    # ---- the reference CPU adapter's per-key shape (one-slice values, 0xFFFFFFFF absent)
+    ABSENT = 0xFFFFFFFF
+
+    def put1(self, keys, maxK, lens, values, n):
+        """Insert that overwrites an existing value (cpu_masstree_adapter.hpp:96-104)."""
+        keys, lens, values = _u32(keys), _u32(lens), _u32(values)
+        self._lib.cpx_put1(self._h, _p(keys), maxK, _p(lens), _p(values), n, self.threads)
+
+    def get1(self, keys, maxK, lens, n):
+        """The first value slice of each key, ABSENT if the key is not stored
+        (cpu_masstree_adapter.hpp:120-125)."""
+        keys, lens = _u32(keys), _u32(lens)
+        out = np.zeros(n, np.uint32)
+        self._lib.cpx_get1(self._h, _p(keys), maxK, _p(lens), _p(out), n, self.threads)
+        return out
+
+
+def result_line(name, mops_runs):
+    """The reference bench's result line, "<op>: <avg> Mop/s (<n>; <run> <run> ... )"
+    (evaluation/universal_bench.cu:65-75), which its evaluation script parses."""
+    avg = sum(mops_runs) / len(mops_runs)
+    return f"{name}: {avg:g} Mop/s ({len(mops_runs)}; " + "".join(f"{m:g} " for m in mops_runs) + ")"

@@ -302,3 +302,66 @@ def test_keys_of_2_15_slices_or_more():
             _eq(a, b)
     ok, *_, msg = g.validate()
     assert ok, msg
+
+
+@pytest.mark.parametrize("tile", [16, 32])
+def test_churn_keeps_the_pool_bounded(tile):
+    """Rounds that erase every key under half of the live prefixes and insert keys under
+    as many new prefixes (plain erase and insert batches, and mixed batches): the layers
+    of a retired prefix empty out and are unlinked, and their nodes are retired with the
+    records, so the slabs in use stay bounded over the rounds (kept, every round would
+    leave a retired prefix's empty layers behind).  Every result and the final entries
+    equal the oracle's, and the structure validates."""
+    rng = np.random.default_rng(55)
+    n_pre, per, mk, mv = 2000, 6, 6, 2
+
+    def keys_under(prefixes):
+        n = len(prefixes) * per
+        k = np.zeros((n, mk), np.uint32)
+        k[:, :3] = np.repeat(prefixes, per, axis=0)
+        k[:, 3:] = rng.integers(0, 1 << 32, size=(n, 3), dtype=np.uint64).astype(np.uint32)
+        k[:, 3] %= 3  # a layer under each prefix slice, and under each of its key[3] values
+        lens = rng.integers(4, mk + 1, size=n).astype(np.uint32)
+        k[np.arange(mk)[None, :] >= lens[:, None]] = 0
+        _, first = np.unique(k, axis=0, return_index=True)
+        first = np.sort(first)
+        owner = np.repeat(np.arange(len(prefixes)), per)[first]
+        return k[first], lens[first], owner
+
+    g, o = S.GpuBackend(tile, pool_bytes=256 << 20), S.OracleBackend()
+    pre = rng.integers(0, 1 << 32, size=(n_pre, 3), dtype=np.uint64).astype(np.uint32)
+    keys, lens, owner = keys_under(pre)
+    vals = rng.integers(0, 1 << 32, size=(len(lens), mv), dtype=np.uint64).astype(np.uint32)
+    _eq(g.insert(keys, lens, vals, None, mk, mv), o.insert(keys, lens, vals, None, mk, mv))
+    used = []
+    for rnd in range(8):
+        old = rng.permutation(n_pre)[: n_pre // 2]  # prefixes whose keys all go
+        gone = np.isin(owner, old)
+        e = np.nonzero(gone)[0]
+        newp = rng.integers(0, 1 << 32, size=(len(old), 3), dtype=np.uint64).astype(np.uint32)
+        nk, nl, nown = keys_under(newp)
+        nv = rng.integers(0, 1 << 32, size=(len(nl), mv), dtype=np.uint64).astype(np.uint32)
+        if rnd % 2 == 0:
+            _eq(g.erase(keys[e], lens[e], mk), o.erase(keys[e], lens[e], mk))
+            _eq(g.insert(nk, nl, nv, None, mk, mv), o.insert(nk, nl, nv, None, mk, mv))
+        else:
+            types = np.concatenate([np.ones(len(e), np.uint8), np.zeros(len(nl), np.uint8)])
+            perm = rng.permutation(len(types))
+            mkeys = np.concatenate([keys[e], nk])[perm]
+            mlens = np.concatenate([lens[e], nl])[perm]
+            mvals = np.concatenate([vals[e], nv])[perm]
+            vl = np.full(len(types), mv, np.uint32)
+            for a, b in zip(g.mixed(types[perm], mkeys, mlens, mvals, vl, mk, mv),
+                            o.mixed(types[perm], mkeys, mlens, mvals, vl, mk, mv)):
+                _eq(a, b)
+        # surviving prefixes keep their slots; the new ones take the retired slots
+        pre[old] = newp
+        keys = np.concatenate([keys[~gone], nk])
+        lens = np.concatenate([lens[~gone], nl])
+        vals = np.concatenate([vals[~gone], nv])
+        owner = np.concatenate([owner[~gone], old[nown]])
+        used.append(g.t.slabs_allocated())
+    assert g.items() == o.items()
+    ok, *_, msg = g.validate()
+    assert ok, msg
+    assert max(used[4:]) <= 1.1 * max(used[:2]), used
