@@ -71,6 +71,19 @@ int gmt_find(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
              const uint32_t* key_lengths, uint32_t* values, uint32_t max_value_length,
              uint32_t* value_lengths, uint32_t num_keys, int concurrent, void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * gmt_find with concurrent = 0, whose per-key scratch of the two-pass find is a fixed
+ * device array of GMT_NOALLOC_MAX_KEYS words inside the library (one per GPU, held until
+ * the process ends) instead of a buffer the index grows with hipMalloc.  It allocates
+ * nothing; num_keys above GMT_NOALLOC_MAX_KEYS is rejected with GMT_EINVAL.  Every call
+ * on one GPU shares that array, so calls that overlap on one GPU (other streams, other
+ * indexes) race: order them. */
+#define GMT_NOALLOC_MAX_KEYS (1u << 22)
+int gmt_find_noalloc(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
+                     const uint32_t* key_lengths, uint32_t* values, uint32_t max_value_length,
+                     uint32_t* value_lengths, uint32_t num_keys, void* stream);
+
 /* gpu_masstree::insert<update_if_exists> (include/gpu_masstree.hpp:101-115).
  * results (nullable, one byte per request) receives per-op success: an extension. */
 int gmt_insert(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,

@@ -157,6 +157,56 @@ struct gpu_masstree {
     }
   }
 
+  // find<false> whose two-pass scratch (one word per key) is the caller's `scratch` of
+  // `scratch_words` words instead of the index's own buffer: nothing is allocated, so
+  // it may run where allocation is not possible (graph capture, serving loops).  The
+  // caller keeps num_keys <= scratch_words; calls sharing one scratch must not overlap.
+  void find_with_scratch(const key_slice_type* keys, const size_type max_key_length,
+                         const size_type* key_lengths, value_slice_type* values,
+                         const size_type max_value_length, size_type* value_lengths,
+                         const size_type num_keys, uint32_t* scratch, uint64_t scratch_words,
+                         hipStream_t stream = 0) {
+    if (num_keys > scratch_words || !key_length_ok(max_key_length, stream)) return;
+    indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
+    a.kind = indexingpu_amd::op_find;
+    a.max_value_length = max_value_length;
+    a.out_values = values;
+    a.out_value_lengths = value_lengths;
+    switch (find_tile_) {
+      case 1: launch_narrow<1>(a, stream, false, 0, 0, scratch); break;
+      case 4: launch_narrow<4>(a, stream, false, 0, 0, scratch); break;
+      case 8: launch_narrow<8>(a, stream, false, 0, 0, scratch); break;
+      case 16: launch_narrow<16>(a, stream, false, 0, 0, scratch); break;
+      case 2: launch_narrow<2>(a, stream, false, 0, 0, scratch); break;
+      default: launch_wide(a, stream); break;
+    }
+  }
+  // Fills the per-kernel occupancy cache of every read-only find kernel, so that a
+  // later find_with_scratch makes no host allocation either.
+  void warm_find_launches() {
+    using namespace indexingpu_amd;
+    auto warm_t = [](auto tag) {
+      constexpr uint32_t T = decltype(tag)::value;
+      resident_blocks(narrow_descend_kernel<T, 1>);
+      resident_blocks(narrow_descend_kernel<T, 2>);
+      resident_blocks(narrow_descend_kernel<T, 4>);
+      resident_blocks(narrow_record_kernel<T>);
+      resident_blocks(narrow_find_kernel<T, 1>);
+      resident_blocks(narrow_find_kernel<T, 2>);
+      resident_blocks(narrow_find_kernel<T, 4>);
+      resident_blocks(narrow_find_kernel<T, 8>);
+    };
+    warm_t(std::integral_constant<uint32_t, 1>{});
+    warm_t(std::integral_constant<uint32_t, 2>{});
+    warm_t(std::integral_constant<uint32_t, 4>{});
+    warm_t(std::integral_constant<uint32_t, 8>{});
+    warm_t(std::integral_constant<uint32_t, 16>{});
+    resident_blocks(find_kernel<tile_size, 1>);
+    resident_blocks(find_kernel<tile_size, 2>);
+    resident_blocks(find_kernel<tile_size, 4>);
+    resident_blocks(find_kernel<tile_size, 8>);
+  }
+
   // Lookups kept in flight per tile by the read-only find kernels (1, 2, 4 or 8).
   void set_find_depth(uint32_t k) { find_depth_ = (k == 1 || k == 2 || k == 8) ? k : 4; }
   uint32_t find_depth() const { return find_depth_; }
@@ -806,18 +856,22 @@ struct gpu_masstree {
   }
   template <uint32_t T>
   void launch_narrow(const indexingpu_amd::batch_args& a, hipStream_t stream,
-                     bool two_pass = false, uint64_t work = 0, uint32_t depth = 0) {
+                     bool two_pass = false, uint64_t work = 0, uint32_t depth = 0,
+                     uint32_t* park = nullptr) {
     constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / T;
     if (work == 0) work = a.n;
     if (depth == 0) depth = find_depth_;
     if ((two_pass || find_passes_ == 2) && a.n > 0) {
-      ensure_park(a.range ? work : a.n);  // indexed from the first request of the range
+      if (!park) {
+        ensure_park(a.range ? work : a.n);  // indexed from the first request of the range
+        park = park_;
+      }
       auto launch_descend = [&](auto kernel, uint32_t per_block) {
         const uint64_t want = (work + per_block - 1) / per_block;
         const uint32_t blocks =
             (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(kernel));
         hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
-                           view(), a, park_);
+                           view(), a, park);
       };
       // plain find batches take the find-only kernels; the finds pass of a mixed batch
       // (kinds, ranges, write hints) the general ones
@@ -836,11 +890,11 @@ struct gpu_masstree {
         }
       }
       switch (record_tile_ ? record_tile_ : T) {
-        case 1: launch_record<1>(a, stream, work); break;
-        case 2: launch_record<2>(a, stream, work); break;
-        case 4: launch_record<4>(a, stream, work); break;
-        case 8: launch_record<8>(a, stream, work); break;
-        default: launch_record<16>(a, stream, work); break;
+        case 1: launch_record<1>(a, stream, work, park); break;
+        case 2: launch_record<2>(a, stream, work, park); break;
+        case 4: launch_record<4>(a, stream, work, park); break;
+        case 8: launch_record<8>(a, stream, work, park); break;
+        default: launch_record<16>(a, stream, work, park); break;
       }
       return;
     }
@@ -852,13 +906,14 @@ struct gpu_masstree {
     }
   }
   template <uint32_t T>
-  void launch_record(const indexingpu_amd::batch_args& a, hipStream_t stream, uint64_t work) {
+  void launch_record(const indexingpu_amd::batch_args& a, hipStream_t stream, uint64_t work,
+                     uint32_t* park) {
     constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / T;
     auto rec = indexingpu_amd::narrow_record_kernel<T>;
     const uint64_t want = (work + tiles - 1) / tiles;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(rec));
     hipLaunchKernelGGL(rec, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream, view(),
-                       a, static_cast<const uint32_t*>(park_));
+                       a, static_cast<const uint32_t*>(park));
   }
   void launch_wide(const indexingpu_amd::batch_args& a, hipStream_t stream) {
     constexpr uint32_t tiles = indexingpu_amd::kBlockThreads / tile_size;

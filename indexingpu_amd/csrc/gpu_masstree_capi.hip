@@ -4,6 +4,7 @@
 // tile width), so the active width is a run-time choice.
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "gpu_masstree_c.h"
@@ -81,6 +82,39 @@ int guarded(F&& f) {
 }
 
 inline gmt_impl* B(gmt_index* h) { return reinterpret_cast<gmt_impl*>(h); }
+
+// Scratch of gmt_find_noalloc: the two-pass find's per-key word for up to
+// GMT_NOALLOC_MAX_KEYS keys, a fixed device array (each GPU the library runs on holds its
+// own copy until the process ends).  Host code never names the array itself in a
+// kernel argument or copy: its device address, one per device, comes from
+// hipGetSymbolAddress at set-up (gmt_create on that device, or the first
+// gmt_find_noalloc there).
+static __device__ __align__(256) uint32_t scratch[GMT_NOALLOC_MAX_KEYS];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (set up once per device); null on
+// failure, with the error message set.
+uint32_t* noalloc_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    g_last_error = "gmt_find_noalloc: no usable current device";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_last_error = std::string("gmt_find_noalloc: hipGetSymbolAddress failed: ") +
+                     hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 int bad(const char* msg) {
@@ -100,8 +134,17 @@ int gmt_create(uint32_t tile_size, uint64_t pool_bytes, float pool_ratio,
   *out = nullptr;
   if (!valid_tile(tile_size)) return bad("gmt_create: tile_size must be 16, 32 or 64");
   gmt_impl* p = nullptr;
-  int rc = guarded([&] { p = new gmt_impl(tile_size, pool_bytes, pool_ratio, retire_capacity); });
-  if (rc != GMT_OK) return rc;
+  int rc = guarded([&] {
+    p = new gmt_impl(tile_size, pool_bytes, pool_ratio, retire_capacity);
+    // set-up of gmt_find_noalloc on this device: its scratch address and the launch
+    // parameters of the find kernels, so that the entry point itself allocates nothing
+    if (!noalloc_scratch()) throw std::runtime_error(g_last_error);
+    p->on_all([](auto& t) { t.warm_find_launches(); });
+  });
+  if (rc != GMT_OK) {
+    delete p;
+    return rc;
+  }
   *out = reinterpret_cast<gmt_index*>(p);
   return GMT_OK;
 }
@@ -125,6 +168,25 @@ int gmt_find(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
       else
         t.template find<false>(keys, max_key_length, key_lengths, values, max_value_length,
                                value_lengths, num_keys, S(stream));
+    });
+  });
+}
+
+int gmt_find_noalloc(gmt_index* index, const uint32_t* keys, uint32_t max_key_length,
+                     const uint32_t* key_lengths, uint32_t* values, uint32_t max_value_length,
+                     uint32_t* value_lengths, uint32_t num_keys, void* stream) {
+  if (!index) return bad("gmt_find_noalloc: null handle");
+  if (num_keys && (!keys || !value_lengths)) return bad("gmt_find_noalloc: null keys or value_lengths");
+  if (num_keys && max_value_length && !values) return bad("gmt_find_noalloc: null values");
+  if (num_keys > GMT_NOALLOC_MAX_KEYS)
+    return bad("gmt_find_noalloc: num_keys above GMT_NOALLOC_MAX_KEYS");
+  if (max_key_length > 65536) return bad("gmt_find_noalloc: max_key_length above 65536");
+  uint32_t* sp = noalloc_scratch();
+  if (!sp) return GMT_EHIP;
+  return guarded([&] {
+    B(index)->with_tree([&](auto& t) {
+      t.find_with_scratch(keys, max_key_length, key_lengths, values, max_value_length,
+                          value_lengths, num_keys, sp, GMT_NOALLOC_MAX_KEYS, S(stream));
     });
   });
 }

@@ -52,6 +52,7 @@ def load_library():
     lib.gmt_create.argtypes = [u32, u64, ctypes.c_float, u32, ctypes.POINTER(vp)]
     lib.gmt_destroy.argtypes = [vp]
     lib.gmt_find.argtypes = [vp, P, u32, P, P, u32, P, u32, i32, vp]
+    lib.gmt_find_noalloc.argtypes = [vp, P, u32, P, P, u32, P, u32, vp]
     lib.gmt_insert.argtypes = [vp, P, u32, P, P, u32, P, u32, i32, P, vp]
     lib.gmt_update.argtypes = [vp, P, u32, P, P, u32, P, u32, P, vp]
     lib.gmt_erase.argtypes = [vp, P, u32, P, u32, P, vp]
@@ -151,6 +152,20 @@ class GpuMasstree:
             _ptr(values, 4, "values", n * max_value_length), max_value_length,
             _ptr(value_lengths, 4, "value_lengths", n), n, int(bool(concurrent)),
             _stream(stream)))
+
+    # Largest batch of find_noalloc (GMT_NOALLOC_MAX_KEYS).
+    NOALLOC_MAX_KEYS = 1 << 22
+
+    def find_noalloc(self, keys, max_key_length, key_lengths, values, max_value_length,
+                     value_lengths, num_keys, stream=None):
+        """find() (read-only form) that allocates nothing: its scratch is a fixed array in
+        the library (NOALLOC_MAX_KEYS keys at most; calls overlapping on one GPU race)."""
+        n = int(num_keys)
+        self._check(self._lib.gmt_find_noalloc(
+            self._h, _ptr(keys, 4, "keys", n * max_key_length), max_key_length,
+            _ptr(key_lengths, 4, "key_lengths", n),
+            _ptr(values, 4, "values", n * max_value_length), max_value_length,
+            _ptr(value_lengths, 4, "value_lengths", n), n, _stream(stream)))
 
     def insert(self, keys, max_key_length, key_lengths, values, max_value_length, value_lengths,
                num_keys, update_if_exists=False, results=None, stream=None):
