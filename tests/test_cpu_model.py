@@ -150,8 +150,9 @@ def test_split_parent_out_of_memory(nthreads):
     assert b.items() == o.items()
 
 
+@pytest.mark.parametrize("hinted", [False, True], ids=["from_root", "start_hints"])
 @pytest.mark.parametrize("nthreads", [0, 8])
-def test_random_out_of_memory(nthreads):
+def test_random_out_of_memory(nthreads, hinted):
     """One allocation in seven fails, wherever it is made (records, border splits, layer
     push-downs, parent splits): every insert either succeeds or reports false and leaves
     no trace; the tree stays valid and equals the oracle fed only the successful inserts."""
@@ -162,7 +163,7 @@ def test_random_out_of_memory(nthreads):
     keys, lens = keys[np.sort(first)], lens[np.sort(first)]
     vl = rng.integers(1, 4, size=len(lens)).astype(np.uint32)
     vals = rng.integers(0, 1 << 32, size=(len(lens), 3), dtype=np.uint64).astype(np.uint32)
-    b = S.CpuModelBackend(nthreads=nthreads, allow_error=True)
+    b = S.CpuModelBackend(nthreads=nthreads, allow_error=True, hinted=hinted)
     o = S.OracleBackend()
     mk, mv = keys.shape[1], 3
     b.m.set_fail(any_every=7)
@@ -217,9 +218,10 @@ def test_inserts_without_suffix_records(nthreads):
         assert b.items() == o.items()
 
 
+@pytest.mark.parametrize("hinted", [False, True], ids=["from_root", "start_hints"])
 @pytest.mark.parametrize("no_suffix", [False, True], ids=["suffix_records", "layer_chains"])
 @pytest.mark.parametrize("nthreads", [0, 8])
-def test_erase_removes_empty_layers(nthreads, no_suffix):
+def test_erase_removes_empty_layers(nthreads, no_suffix, hinted):
     """Erasing every key under a shared prefix empties the layers below it: each emptied
     layer root is unlinked from the layer above and retired (cascading up through layer
     chains), while inserts into the same prefixes run in the same batches.  Results equal
@@ -239,7 +241,7 @@ def test_erase_removes_empty_layers(nthreads, no_suffix):
     pid = np.array([np.nonzero((pre == k[:3]).all(1))[0][0] for k in keys])
     vals = rng.integers(0, 1 << 32, size=(len(lens), 2), dtype=np.uint64).astype(np.uint32)
     mk, mv = 6, 2
-    b = S.CpuModelBackend(nthreads=nthreads, no_suffix=no_suffix)
+    b = S.CpuModelBackend(nthreads=nthreads, no_suffix=no_suffix, hinted=hinted)
     o = S.OracleBackend()
     _eq = np.array_equal
     assert _eq(b.insert(keys, lens, vals, None, mk, mv), o.insert(keys, lens, vals, None, mk, mv))
@@ -265,3 +267,46 @@ def test_erase_removes_empty_layers(nthreads, no_suffix):
     # everything erased: only the top layer is left
     assert _eq(b.erase(keys, lens, mk), o.erase(keys, lens, mk))
     assert b.items() == [] and b.m.layers == 1
+
+
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_hinted_writes_on_few_hot_nodes(nthreads):
+    """Hinted writes (the GPU write passes' fast path: load, compare-and-swap, publish as
+    separate steps) crowded onto a few hundred keys: every kind of write, upserts and
+    updates replacing values of every length (inline, record, suffix record), erases,
+    re-inserts, with most lock attempts failing and going round again.  Results equal the
+    oracle's batch after batch."""
+    rng = np.random.default_rng(61)
+    keys, lens = W.unique_random_keys(rng, 600, 1, 3)
+    keys[:, 0] %= 40  # ~40 first slices: a few border nodes, suffix records, layers
+    _, first = np.unique(keys, axis=0, return_index=True)
+    keys, lens = keys[np.sort(first)], lens[np.sort(first)]
+    n, mk, mv = len(lens), 3, 3
+    b = S.CpuModelBackend(nthreads=nthreads, ntiles=64, hinted=True)
+    o = S.OracleBackend()
+    for rnd in range(12):
+        # each key at most once per batch, every kind
+        idx = rng.permutation(n)[: n // 2]
+        types = rng.choice([0, 1, 3, 2, 0], size=len(idx)).astype(np.uint8)
+        types[types == 2] = 4 if rnd % 2 else 0  # upserts (kind 4) in odd rounds
+        vals = rng.integers(0, 1 << 32, size=(len(idx), mv), dtype=np.uint64).astype(np.uint32)
+        vl = rng.integers(0, mv + 1, size=len(idx)).astype(np.uint32)
+        for kind in (0, 4, 3, 1):
+            sel = idx[types == kind]
+            if not len(sel):
+                continue
+            v, l = vals[types == kind], vl[types == kind]
+            if kind == 0:
+                got = b.insert(keys[sel], lens[sel], v, l, mk, mv)
+                exp = o.insert(keys[sel], lens[sel], v, l, mk, mv)
+            elif kind == 4:
+                got = b.insert(keys[sel], lens[sel], v, l, mk, mv, upsert=True)
+                exp = o.insert(keys[sel], lens[sel], v, l, mk, mv, upsert=True)
+            elif kind == 3:
+                got = b.update(keys[sel], lens[sel], v, l, mk, mv)
+                exp = o.update(keys[sel], lens[sel], v, l, mk, mv)
+            else:
+                got = b.erase(keys[sel], lens[sel], mk)
+                exp = o.erase(keys[sel], lens[sel], mk)
+            assert np.array_equal(got, exp), (rnd, kind)
+        assert b.items() == o.items()
