@@ -257,10 +257,17 @@ struct gpu_masstree {
     launch_hinted(a, stream);
   }
 
-  // Erase removes layer roots it leaves empty (do_remove_empty_root, as the reference);
-  // it does not merge or borrow between nodes of a layer, so do_merge and
-  // pessimistic_merge are accepted and have no effect on structure (results are the
-  // same either way).
+  // Erase folds a border node it leaves empty into a neighbour (do_merge) and removes
+  // layer roots it leaves empty (do_remove_empty_root), as the reference's switches ask;
+  // pessimistic_merge (when the reference merges: on the way down or on a retry) has no
+  // counterpart here, since nodes are merged once they are empty.  Results are the same
+  // under every setting.
+  template <bool do_merge, bool do_remove_empty_root>
+  static constexpr uint32_t erase_flags() {
+    return (do_merge ? 0u : indexingpu_amd::kReqNoMerge) |
+           (do_remove_empty_root ? 0u : indexingpu_amd::kReqKeepEmptyLayers);
+  }
+
   template <bool do_remove_empty_root = true, bool pessimistic_merge = true, bool do_merge = true,
             bool concurrent = true, bool use_shmem_key = false>
   void erase(const key_slice_type* keys, const size_type max_key_length,
@@ -269,7 +276,7 @@ struct gpu_masstree {
     if (!key_length_ok(max_key_length, stream)) return;
     indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_keys);
     a.kind = indexingpu_amd::op_erase;
-    a.flags = do_remove_empty_root ? 0u : indexingpu_amd::kReqKeepEmptyLayers;
+    a.flags = erase_flags<do_merge, do_remove_empty_root>();
     a.results = results;
     launch_hinted(a, stream);
   }
@@ -322,7 +329,7 @@ struct gpu_masstree {
     indexingpu_amd::batch_args a = base_args(keys, max_key_length, key_lengths, num_requests);
     a.types = reinterpret_cast<const uint8_t*>(request_types);
     a.flags = (enable_suffix ? 0u : indexingpu_amd::kReqNoSuffix) |
-              (erase_do_remove_empty_root ? 0u : indexingpu_amd::kReqKeepEmptyLayers);
+              erase_flags<erase_do_merge, erase_do_remove_empty_root>();
     a.kind = indexingpu_amd::op_find;
     a.values = values;
     a.max_value_length = max_value_length;
@@ -570,8 +577,7 @@ struct gpu_masstree {
     c.ac = allocator.cache;
     c.rc = reclaimer.cache;
     indexingpu_amd::request rq{indexingpu_amd::op_erase, key_length, as_ptr(key), nullptr, 0,
-                               nullptr, 0,
-                               do_remove_empty_root ? 0u : indexingpu_amd::kReqKeepEmptyLayers};
+                               nullptr, 0, erase_flags<do_merge, do_remove_empty_root>()};
     const bool r = run_op<true>(c, rq) != 0;
     allocator.cache = c.ac;
     reclaimer.cache = c.rc;

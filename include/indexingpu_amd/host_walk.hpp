@@ -24,6 +24,7 @@ struct walk_stats {
   uint64_t record_slabs = 0;
   uint64_t max_depth = 0;       // deepest B-link level count in any layer
   uint64_t unlinked = 0;        // non-root nodes with no parent slot (reached by sibling links)
+  uint64_t empty_borders = 0;   // non-root border nodes with no entry
   std::string error;            // empty when every invariant holds
 };
 
@@ -119,6 +120,7 @@ class host_walker {
         if (!p) return false;
         uint32_t hh = slot_lo(p[kMetaSlot]);
         if (hdr_border(hh)) return fail("border node in interior level");
+        if (hdr_removed(hh)) return fail("removed node reachable: " + std::to_string(n));
         uint32_t nk = hdr_nkeys(hh);
         if (nk == 0 || nk > kInteriorCap) return fail("bad interior nkeys");
         uint32_t tag = 0;
@@ -132,8 +134,12 @@ class host_walker {
           if (!first && i == 0 && b <= prev_high) return fail("interior bound below left high");
           const uint64_t* c = node(child_of(p[i]));
           if (!c) return false;
-          if (hdr_level(slot_lo(c[kMetaSlot])) + 1 != hdr_level(hh))
-            return fail("child level mismatch");
+          const uint32_t ch = slot_lo(c[kMetaSlot]);
+          if (hdr_level(ch) + 1 != hdr_level(hh)) return fail("child level mismatch");
+          // a child's keys never exceed its slot's bound (its high key, or its last bound)
+          const uint32_t chigh = hdr_border(ch) ? slot_lo(c[kHighSlot])
+                                 : (hdr_nkeys(ch) ? slot_lo(c[hdr_nkeys(ch) - 1]) : 0u);
+          if (chigh > b) return fail("child above its parent bound");
         }
         prev_high = slot_lo(p[nk - 1]);
         first = false;
@@ -160,9 +166,12 @@ class host_walker {
       uint32_t hh = slot_lo(p[kMetaSlot]);
       if (!hdr_border(hh)) return fail("interior node in border level");
       if (hdr_locked(hh)) return fail("border node left locked");
+      if (hdr_removed(hh)) return fail("removed node reachable: " + std::to_string(n));
       uint32_t nk = hdr_nkeys(hh);
       if (nk > kBorderCap) return fail("bad border nkeys");
       uint32_t high = slot_lo(p[kHighSlot]);
+      if (!first && high <= prev_high) return fail("border high not above left sibling's");
+      if (nk == 0 && level_heads.size() > 1) stats_.empty_borders++;
       uint32_t ksb = slot_hi(p[kHighSlot]);
       if (level_heads.size() > 1 && !linked.count(n)) stats_.unlinked++;
       stats_.nodes++;

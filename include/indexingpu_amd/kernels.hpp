@@ -69,7 +69,7 @@ struct batch_args {
   // root.
   u32* hints;
   // kReqNoSuffix (insert / upsert with enable_suffix = false), kReqKeepEmptyLayers
-  // (erase<do_remove_empty_root = false>)
+  // (erase<do_remove_empty_root = false>), kReqNoMerge (erase<..., do_merge = false>)
   u32 flags;
 };
 constexpr u32 kHintWords = 4;

@@ -370,6 +370,22 @@ struct wave_tile {
       n.s = slot_make(hdr_with_nkeys(h, nk + 1), slot_hi(n.s));
     }
   }
+  // removes child slot j (j > 0): slot j - 1 keeps its child and takes slot j's bound
+  __device__ __forceinline__ void interior_remove(node_t& n, u32 j) const {
+    const u32 h = header(n), nk = hdr_nkeys(h);
+    const u32 bj = bcast(slot_lo(n.s), j);
+    const u64 next = slot_from(n.s, slot + 1 < kSlots ? slot + 1 : kSlots - 1);
+    if (slot < kInteriorCap) {
+      if (slot == j - 1) n.s = slot_make(bj, slot_hi(n.s));
+      else if (slot >= j && slot + 1 < nk) n.s = next;
+      else if (slot >= j && slot < nk) n.s = 0;
+    } else {
+      n.s = slot_make(hdr_with_nkeys(h, nk - 1), slot_hi(n.s));
+    }
+  }
+  __device__ __forceinline__ u32 entry_child(const node_t& n, u32 j) const {
+    return bcast(slot_hi(n.s), j) & kChildMask;
+  }
   __device__ __forceinline__ u32 interior_split(const node_t& n, u32 j, u32 sep, u32 lc, u32 rc,
                                                 node_t& left, node_t& right) const {
     const u32 nk = nkeys_of(n), total = nk + 1, m = total / 2;

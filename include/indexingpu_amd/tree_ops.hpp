@@ -59,6 +59,8 @@ enum op_phase : u32 {
   ph_unlink_descend = 6,  // erase emptied layer root a0: walk from the tree root to the
                           // border node of layer a1 that holds its LINK
   ph_unlink_lock = 7,     // lock that node and remove the LINK
+  ph_merge_descend = 8,   // erase emptied border node a0 (held locked): find its parent
+  ph_merge_lock = 9,      // lock the parent and the left sibling, fold a0 into the sibling
 };
 
 // One request, uniform over the tile.
@@ -70,13 +72,15 @@ struct request {
   u32 value_length;
   u32* out_value;       // find output (max_value_length slices)
   u32 max_value_length;
-  u32 flags;  // kReqNoSuffix | kReqKeepEmptyLayers
+  u32 flags;  // kReqNoSuffix | kReqKeepEmptyLayers | kReqNoMerge
 };
 // insert / upsert with enable_suffix = false: a new key's slices past the current layer
 // become one single-entry layer each instead of a suffix record
 constexpr u32 kReqNoSuffix = 1u;
 // erase<do_remove_empty_root = false>: a layer emptied by an erase stays in place
 constexpr u32 kReqKeepEmptyLayers = 2u;
+// erase<..., do_merge = false>: a border node emptied by an erase stays in place
+constexpr u32 kReqNoMerge = 4u;
 
 template <class NodeImage>
 struct op_state {
@@ -91,7 +95,8 @@ struct op_state {
   u32 t0;          // coarse clock when the time guard started (0 = not yet)
   u32 hint[4];     // last interior node visited at levels 1..4 of the current layer
   u32 lvl;         // expected level of `node` (kInvalid: unknown)
-  u32 fresh;       // 1: `snap_hdr` was validated by the step just before this lock attempt
+  u32 fresh;       // bit 0: `snap_hdr` was validated by the step just before this lock
+                   // attempt; bit 1: read interior nodes coherently (after a restart)
   NodeImage img;   // last validated image of `node`
 };
 
@@ -110,20 +115,28 @@ constexpr u32 kHintLevels = 4;
 template <class S>
 IGPU_HD bool op_holds_locks(const S& st) {
   return st.phase == ph_parent_descend || st.phase == ph_parent_lock ||
-         st.phase == ph_unlink_descend || st.phase == ph_unlink_lock;
+         st.phase == ph_unlink_descend || st.phase == ph_unlink_lock ||
+         st.phase == ph_merge_descend || st.phase == ph_merge_lock;
+}
+template <class S>
+IGPU_HD bool op_in_merge(const S& st) {
+  return st.phase == ph_merge_descend || st.phase == ph_merge_lock;
 }
 template <class S>
 IGPU_HD bool op_in_split(const S& st) {
   return st.phase == ph_parent_descend || st.phase == ph_parent_lock;
 }
 
-// Restarts a write from the tree root (it met a layer root that an erase removed).
+// Restarts a write from the tree root (it met a node that an erase removed).  Interior
+// nodes are read coherently from then on: a stale cached interior line could route it to
+// the removed node again.
 template <class S>
 IGPU_HD void op_restart(S& st, u32 root) {
   st.layer = 0;
   st.node = root;
   st.layer_root = root;
   st.snap_hdr = kInvalid;
+  st.fresh |= 2u;
   clear_hints(st);
   st.phase = ph_descend;
 }
@@ -273,6 +286,27 @@ IGPU_HD bool layer_emptied(const request& rq, const S& st, u32 h) {
          !(rq.flags & kReqKeepEmptyLayers);
 }
 
+// An entry was removed from a non-root border node (locked header h before the removal)
+// that is empty now.
+IGPU_HD bool border_emptied(const request& rq, u32 h) {
+  return !hdr_root(h) && hdr_border(h) && hdr_nkeys(h) == 1 && !(rq.flags & kReqNoMerge);
+}
+
+// Starts folding the emptied non-root border node st.node (locked, image n written)
+// into its left sibling (ph_merge_*); it stays locked meanwhile.  The reference merges
+// and borrows between underflowing nodes (include/gpu_masstree.hpp:1151-1330); here a
+// node is merged once it is empty.
+template <class C, class S>
+IGPU_HD void begin_merge(C& c, S& st, const typename C::node_t& n) {
+  c.fence_release();
+  st.a0 = st.node;
+  st.a1 = c.sibling(n);
+  st.a2 = c.high(n);
+  st.a3 = 1;
+  goto_parent(st);
+  st.phase = ph_merge_descend;
+}
+
 // An erase removed an entry from the locked border node st.node (image n, locked header
 // h before the removal): publish it.  A layer root left empty (below the top layer) is
 // marked removed, stays locked, and its LINK is removed from the layer above
@@ -289,8 +323,22 @@ IGPU_HD void erase_publish(C& c, const request& rq, S& st, typename C::node_t& n
     return;
   }
   c.write_node(st.node, n);
+  st.result = 1;
+  if (border_emptied(rq, h)) {
+    begin_merge(c, st, n);
+    return;
+  }
   c.unlock_bump(st.node);
   op_finish(st, 1);
+}
+
+// The merge of an emptied border node gives up (contention, or its parent slot is not
+// where a merge can use it): the node stays, empty and linked, which every reader and
+// writer handles.
+template <class C, class S>
+IGPU_HD void merge_give_up(C& c, S& st) {
+  c.unlock_bump(st.a0);
+  op_finish(st, st.result);
 }
 
 // Returns true when the step made progress (false: retry later).
@@ -313,6 +361,8 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         // one's sibling link, and a later split of it links it from its left neighbour
         c.unlock_bump(st.a0);
         c.unlock_bump(st.a1);
+      } else if (op_in_merge(st)) {
+        c.unlock_bump(st.a0);  // the emptied node stays, empty and linked
       } else if (op_holds_locks(st)) {
         revive_layer_root(c, st.a0);  // the emptied layer stays, live
       }
@@ -328,7 +378,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       // interior nodes (expected level > 0, or unknown at a layer root) may come from
       // the cache; border nodes are always read coherently, since writers lock them by
       // the version they read
-      if (!c.template read_node<Validate>(st.node, n, st.lvl != 0)) return false;
+      if (!c.template read_node<Validate>(st.node, n, st.lvl != 0 && !(st.fresh & 2u))) return false;
       const u32 h = c.header(n);
       if (!hdr_border(h)) {
         if (hdr_level(h) - 1 < kHintLevels) st.hint[hdr_level(h) - 1] = st.node;
@@ -348,9 +398,12 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         return true;
       }
       st.lvl = 0;
-      if (hdr_removed(h) && rq.kind != op_find) {
-        // a layer root an erase removed (reached through a LINK or hint read before):
-        // the key's place is in the layer above now
+      if (hdr_removed(h)) {
+        // a removed node, reached through a pointer read before its removal: an
+        // emptied layer root (the key's place is in the layer above now) or a border
+        // node merged into its left sibling (whose range now covers the key).  Start
+        // again from the tree root, reading interior nodes coherently, so the walk
+        // sees the parent slot the merge rewrote.
         op_restart(st, c.tree_root);
         return true;
       }
@@ -388,7 +441,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         return true;
       }
       st.snap_hdr = h;
-      st.fresh = 1;
+      st.fresh |= 1u;
       st.phase = ph_lock;
       return true;
     }
@@ -421,8 +474,8 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       // past the current right edge of the tree ran 5-10x slower that way).
       // the first attempt right after the validating read goes straight to the
       // compare-and-swap; a retry first checks the header with a plain load
-      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr, st.fresh != 0);
-      st.fresh = 0;
+      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr, (st.fresh & 1u) != 0);
+      st.fresh &= ~1u;
       if (r == kLockBusy) return false;
       if (r == kLockStale) {
         st.phase = ph_descend;
@@ -630,7 +683,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       }
       if (lvl == st.a3) {
         st.snap_hdr = h;
-        st.fresh = 1;
+        st.fresh |= 1u;
         st.phase = ph_parent_lock;
         return true;
       }
@@ -642,8 +695,8 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
     }
 
     case ph_parent_lock: {
-      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr, st.fresh != 0);
-      st.fresh = 0;
+      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr, (st.fresh & 1u) != 0);
+      st.fresh &= ~1u;
       if (r == kLockBusy) return false;
       if (r == kLockStale) {  // re-read the candidate (it may have split or grown)
         st.phase = ph_parent_descend;
@@ -741,7 +794,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
     case ph_unlink_descend: {
       // the same B-link descent as ph_descend, through LINK entries down to layer a1
       node_t& n = st.img;
-      if (!c.template read_node<Validate>(st.node, n, st.lvl != 0)) return false;
+      if (!c.template read_node<Validate>(st.node, n, st.lvl != 0 && !(st.fresh & 2u))) return false;
       const u32 h = c.header(n);
       if (!hdr_border(h)) {
         u32 child = c.find_child(n, q);
@@ -761,18 +814,27 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         return true;
       }
       st.lvl = 0;
+      if (hdr_removed(h)) {
+        // a border node merged away (its entries live in its neighbour now), reached
+        // through a pointer read before the merge: walk again from the tree root
+        const u32 target = st.a1;
+        op_restart(st, c.tree_root);
+        st.a1 = target;
+        st.phase = ph_unlink_descend;
+        return true;
+      }
       if (q > c.high(n) && c.sibling(n) != kInvalid) {
         st.node = c.sibling(n);
         return true;
       }
       if (st.layer == st.a1) {
         st.snap_hdr = h;
-        st.fresh = 1;
+        st.fresh |= 1u;
         st.phase = ph_unlink_lock;
         return true;
       }
       u32 ks = 0, val = 0;
-      if (c.lookup(n, q, true, ks, val) < 0 || ks != kLink || hdr_removed(h)) {
+      if (c.lookup(n, q, true, ks, val) < 0 || ks != kLink) {
         // the path to the emptied layer is gone (not expected: only the holder of a
         // layer root's lock removes its LINK, and a layer with a LINK is not empty)
         revive_layer_root(c, st.a0);
@@ -788,8 +850,8 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
     }
 
     case ph_unlink_lock: {
-      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr, st.fresh != 0);
-      st.fresh = 0;
+      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr, (st.fresh & 1u) != 0);
+      st.fresh &= ~1u;
       if (r == kLockBusy) return false;
       if (r == kLockStale) {
         st.phase = ph_unlink_descend;
@@ -825,9 +887,127 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         return true;
       }
       c.write_node(st.node, n);
-      c.unlock_bump(st.node);
       c.unlock_bump(removed);  // stays marked removed: later visitors restart
       c.retire(removed | kRetiredNode);
+      if (border_emptied(rq, h)) {
+        begin_merge(c, st, n);
+        return true;
+      }
+      c.unlock_bump(st.node);
+      op_finish(st, st.result);
+      return true;
+    }
+
+    case ph_merge_descend: {
+      // B-link descent to the level-1 node covering the emptied node's high key (a2),
+      // from the level-1 hint of the erase, else the layer root
+      node_t& n = st.img;
+      if (!c.template read_node<true>(st.node, n)) return false;
+      const u32 h = c.header(n);
+      const u32 lvl = hdr_level(h);
+      if (hdr_border(h) || lvl < st.a3 || hdr_removed(h)) {
+        merge_give_up(c, st);  // the layer has one level (no parent), or a stale hint
+        return true;
+      }
+      if (lvl == st.a3) {
+        st.snap_hdr = h;
+        st.fresh |= 1u;
+        st.phase = ph_merge_lock;
+        return true;
+      }
+      if (lvl - 1 < kHintLevels) st.hint[lvl - 1] = st.node;
+      u32 child = c.find_child(n, st.a2);
+      if (child == kInvalid) child = c.sibling(n);
+      if (child == kInvalid) {
+        merge_give_up(c, st);
+        return true;
+      }
+      st.node = child;
+      return true;
+    }
+
+    case ph_merge_lock: {
+      const u32 r = c.try_lock_snapshot(st.node, st.snap_hdr, (st.fresh & 1u) != 0);
+      st.fresh &= ~1u;
+      if (r == kLockBusy) return false;
+      if (r == kLockStale) {
+        st.phase = ph_merge_descend;
+        return false;
+      }
+      node_t& n = st.img;
+      const u32 h = st.snap_hdr | kLockBit;
+      c.set_header(n, h);
+      u32 j;
+      if (!c.find_child_slot(n, st.a0, j)) {
+        u32 jc, cc;
+        if (!c.find_cover(n, st.a2, jc, cc) && c.sibling(n) != kInvalid) {
+          // the slot moved right with a split of this node
+          const u32 sib = c.sibling(n);
+          c.unlock_same(st.node);
+          st.node = sib;
+          st.phase = ph_merge_descend;
+          return true;
+        }
+        c.unlock_same(st.node);  // no slot for it (linked from its left neighbour)
+        merge_give_up(c, st);
+        return true;
+      }
+      // Merge with the neighbour under the same parent: the left one (slot j - 1), or,
+      // for the parent's first child, the right one (slot 1).  Of the two nodes, the
+      // left one stays: it takes the right one's range, sibling link and (when the
+      // emptied node is the left one) entries; the parent drops the right one's slot,
+      // whose bound goes to the left one's; the right one is marked removed and retired.
+      if (j == 0 && hdr_nkeys(h) < 2) {
+        c.unlock_same(st.node);  // the parent's only child: leave it
+        merge_give_up(c, st);
+        return true;
+      }
+      // The neighbour gets one lock attempt: splits lock it before this parent, so
+      // waiting for it here could deadlock; a busy neighbour ends the merge.
+      const u32 other = c.entry_child(n, j ? j - 1 : 1);
+      u32 oold;
+      if (!c.try_lock(other, oold)) {
+        c.unlock_same(st.node);
+        merge_give_up(c, st);
+        return true;
+      }
+      node_t on;
+      c.read_node_locked(other, on);
+      if (!hdr_border(oold) || hdr_removed(oold) ||
+          (j ? c.sibling(on) != st.a0 : other != st.a1)) {
+        c.unlock_same(other);  // not the adjacent border node (a split is linking one)
+        c.unlock_same(st.node);
+        merge_give_up(c, st);
+        return true;
+      }
+      node_t en;
+      c.read_node_locked(st.a0, en);
+      const u32 eh = c.header(en);
+      u32 left = other, right = st.a0;
+      if (j) {
+        c.set_header(on, oold | kLockBit);
+        c.set_sibling(on, st.a1);
+        c.set_high(on, st.a2);
+        c.write_node(left, on);
+        c.set_header(en, hdr_remove(eh));
+        c.write_node(right, en);
+      } else {
+        // the emptied node takes the right neighbour's image (entries, high key,
+        // keystates, sibling link) under its own header
+        left = st.a0;
+        right = other;
+        c.set_header(on, hdr_with_nkeys(eh, hdr_nkeys(oold)));
+        c.write_node(left, on);
+        c.set_header(on, hdr_remove(oold | kLockBit));
+        c.write_node(right, on);
+      }
+      c.interior_remove(n, j ? j : 1);
+      c.write_node(st.node, n);
+      c.fence_release();
+      c.unlock_bump(left);
+      c.unlock_bump(st.node);
+      c.unlock_bump(right);
+      c.retire(right | kRetiredNode);
       op_finish(st, st.result);
       return true;
     }

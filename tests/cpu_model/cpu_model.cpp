@@ -25,6 +25,7 @@ struct model {
   u32 root;
   u32 hinted = 0;  // writes start at hints from a read-only walk (as the GPU write passes)
   u32 no_suffix = 0;  // inserts build layers instead of suffix records (enable_suffix=false)
+  u32 erase_flags = 0;  // kReqNoMerge | kReqKeepEmptyLayers for every request
   explicit model(u32 n) : pool(n) {
     cpu_tile t(pool);
     root = t.alloc_slab();
@@ -49,6 +50,7 @@ struct batch_args {
   uint32_t* out_vlens;
   uint8_t* results;
   uint32_t no_suffix;
+  uint32_t erase_flags;
 };
 
 request make_request(const batch_args& a, uint64_t i, uint32_t* scratch_out) {
@@ -58,7 +60,7 @@ request make_request(const batch_args& a, uint64_t i, uint32_t* scratch_out) {
   rq.key = a.keys + i * (uint64_t)a.maxK;
   rq.value = a.values ? a.values + i * (uint64_t)a.maxV : nullptr;
   rq.value_length = a.vlens ? a.vlens[i] : a.maxV;
-  rq.flags = a.no_suffix ? kReqNoSuffix : 0u;
+  rq.flags = (a.no_suffix ? kReqNoSuffix : 0u) | a.erase_flags;
   rq.out_value = a.out_values ? a.out_values + i * (uint64_t)a.maxV : scratch_out;
   rq.max_value_length = a.out_values ? a.maxV : 0;
   // a find in a mixed batch writes its value over the request's value slots
@@ -159,6 +161,7 @@ uint32_t cm_slabs_used(void* h) { return static_cast<model*>(h)->pool.next.load(
 uint64_t cm_retired(void* h) { return static_cast<model*>(h)->pool.retired.size(); }
 void cm_set_hinted(void* h, uint32_t on) { static_cast<model*>(h)->hinted = on; }
 void cm_set_no_suffix(void* h, uint32_t on) { static_cast<model*>(h)->no_suffix = on; }
+void cm_set_erase_flags(void* h, uint32_t f) { static_cast<model*>(h)->erase_flags = f; }
 // Allocation failure injection (0 = off): every k-th allocation, and every k-th
 // allocation of a split inserting into its parent.  Also clears the error word.
 void cm_set_fail(void* h, uint32_t any_every, uint32_t parent_every) {
@@ -183,6 +186,15 @@ uint64_t cm_unlinked(void* h) {
   }, nullptr);
   return w.run(M.root).unlinked;
 }
+// Reachable nodes (which = 0), border nodes (1), empty non-root border nodes (2).
+uint64_t cm_nodes(void* h, uint32_t which) {
+  model& M = *static_cast<model*>(h);
+  host_walker w([&](uint32_t idx) -> const uint64_t* {
+    return idx < M.pool.num_slabs ? M.pool.slab(idx) : nullptr;
+  }, nullptr);
+  const walk_stats s = w.run(M.root);
+  return which == 0 ? s.nodes : which == 1 ? s.border_nodes : s.empty_borders;
+}
 
 // Runs one batch.  nthreads == 0: interleaved simulation with ntiles tiles and the
 // given scheduler seed; nthreads > 0: that many host threads (static partition).
@@ -193,7 +205,7 @@ void cm_batch(void* h, const uint8_t* kinds, uint32_t default_kind, const uint32
               uint64_t seed) {
   model& M = *static_cast<model*>(h);
   batch_args a{kinds, default_kind, keys, maxK, klens, values, maxV, vlens,
-               out_values, out_vlens, results, M.no_suffix};
+               out_values, out_vlens, results, M.no_suffix, M.erase_flags};
   const std::vector<start_hint> hs = batch_hints(M, a, n);
   if (nthreads == 0) {
     std::mt19937_64 rng(seed);

@@ -270,6 +270,14 @@ struct cpu_tile {
     n.s[j + 1] = slot_make(old_bound, right);
     set_header(n, hdr_with_nkeys(header(n), nk + 1));
   }
+  void interior_remove(node_t& n, u32 j) {
+    u32 nk = nkeys_of(n);
+    n.s[j - 1] = slot_make(slot_lo(n.s[j]), slot_hi(n.s[j - 1]));
+    for (u32 i = j; i + 1 < nk; i++) n.s[i] = n.s[i + 1];
+    n.s[nk - 1] = 0;
+    set_header(n, hdr_with_nkeys(header(n), nk - 1));
+  }
+  u32 entry_child(const node_t& n, u32 j) const { return child_of(n.s[j]); }
   u32 interior_split(const node_t& n, u32 j, u32 sep, u32 lchild, u32 rchild, node_t& left,
                      node_t& right) {
     u32 nk = nkeys_of(n);

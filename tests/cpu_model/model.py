@@ -30,8 +30,11 @@ def _load():
         lib.cm_set_hinted.argtypes = [vp, u32]
         lib.cm_set_fail.argtypes = [vp, u32, u32]
         lib.cm_set_no_suffix.argtypes = [vp, u32]
+        lib.cm_set_erase_flags.argtypes = [vp, u32]
         lib.cm_unlinked.argtypes = [vp]
         lib.cm_unlinked.restype = u64
+        lib.cm_nodes.argtypes = [vp, u32]
+        lib.cm_nodes.restype = u64
         lib.cm_layers.argtypes = [vp]
         lib.cm_layers.restype = u64
         lib.cm_batch.argtypes = [vp, u8p, u32, u32p, u32, u32p, u32p, u32, u32p, u32p, u32p,
@@ -71,6 +74,11 @@ class CpuModel:
         suffix record (insert<..., enable_suffix = false>)."""
         self.lib.cm_set_no_suffix(self.h, 1 if on else 0)
 
+    def set_erase_flags(self, flags):
+        """Request flags for every request: 4 = no merging of emptied border nodes
+        (do_merge = false), 2 = emptied layers stay (do_remove_empty_root = false)."""
+        self.lib.cm_set_erase_flags(self.h, flags)
+
     def set_fail(self, any_every=0, parent_every=0):
         """Injected allocation failures: every k-th allocation fails, and every k-th
         allocation of a split inserting into its parent (0 = off).  Clears the error."""
@@ -85,6 +93,10 @@ class CpuModel:
     def retired(self):
         """Objects retired so far (records; nodes carry bit 31)."""
         return int(self.lib.cm_retired(self.h))
+
+    def nodes(self, which=0):
+        """Reachable nodes (0), border nodes (1), or empty non-root border nodes (2)."""
+        return int(self.lib.cm_nodes(self.h, which))
 
     @property
     def unlinked(self):

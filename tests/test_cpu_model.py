@@ -310,3 +310,68 @@ def test_hinted_writes_on_few_hot_nodes(nthreads):
                 exp = o.erase(keys[sel], lens[sel], mk)
             assert np.array_equal(got, exp), (rnd, kind)
         assert b.items() == o.items()
+
+
+@pytest.mark.parametrize("hinted", [False, True], ids=["from_root", "start_hints"])
+@pytest.mark.parametrize("merge", [True, False], ids=["merge", "no_merge"])
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_erase_merges_empty_border_nodes(nthreads, hinted, merge):
+    """Erases that empty non-root border nodes fold them into their left sibling (the
+    sibling takes the node's key range and sibling link, the parent drops its slot, the
+    node is retired), while inserts, finds and other erases run in the same batches over
+    the same ranges.  Results equal the oracle's batch after batch; the walk finds the
+    structure valid and no removed node reachable; the node count falls with the keys.
+    With erase<..., do_merge = false> (and do_remove_empty_root = false, which the
+    reference requires with it) every node stays, with the same results."""
+    rng = np.random.default_rng(71)
+    n, mk, mv = 6000, 2, 1
+    keys, lens = W.unique_random_keys(rng, n, 1, 2)
+    keys[:, 0] %= 3000  # ~2 keys per first slice: a few layers below, many border nodes
+    _, first = np.unique(keys, axis=0, return_index=True)
+    keys, lens = keys[np.sort(first)], lens[np.sort(first)]
+    n = len(lens)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    b = S.CpuModelBackend(nthreads=nthreads, ntiles=64, hinted=hinted)
+    if not merge:
+        b.m.set_erase_flags(4 | 2)
+    o = S.OracleBackend()
+    _eq = np.array_equal
+    assert _eq(b.insert(keys, lens, vals, None, mk, mv), o.insert(keys, lens, vals, None, mk, mv))
+    full, full_border = b.m.nodes(0), b.m.nodes(1)
+    # erase whole key ranges (every key with first slice in [lo, hi)) and a random share
+    # elsewhere; finds of erased and kept keys and re-inserts of other erased keys mixed in
+    gone = ((keys[:, 0] >= 500) & (keys[:, 0] < 2500)) | (rng.random(n) < 0.3)
+    e = np.nonzero(gone)[0]
+    assert _eq(b.erase(keys[e], lens[e], mk), o.erase(keys[e], lens[e], mk))
+    assert b.items() == o.items()
+    if merge:
+        assert b.m.nodes(0) < full // 2 and b.m.nodes(1) < full_border // 2
+        # an emptied node stays only as its parent's one child (or when its neighbour was busy)
+        assert b.m.nodes(2) <= b.m.nodes(0) - b.m.nodes(1)
+    else:
+        assert b.m.nodes(0) == full and b.m.nodes(2) > 0
+    for rnd in range(3):
+        back = rng.permutation(e)[: len(e) // 4]
+        rest = np.setdiff1d(np.arange(n), e)
+        out = rng.permutation(rest)[: len(rest) // 3]
+        probe = rng.permutation(n)[:500]
+        probe = np.setdiff1d(probe, np.concatenate([back, out]))
+        types = np.concatenate([np.zeros(len(back), np.uint8), np.ones(len(out), np.uint8),
+                                np.full(len(probe), 2, np.uint8)])
+        idx = np.concatenate([back, out, probe])
+        p = rng.permutation(len(idx))
+        types, idx = types[p], idx[p]
+        v = vals[idx].copy()
+        vl = np.full(len(idx), mv, np.uint32)
+        for g, x in zip(b.mixed(types, keys[idx], lens[idx], v, vl, mk, mv),
+                        o.mixed(types, keys[idx], lens[idx], v.copy(), vl, mk, mv)):
+            assert _eq(g, x), rnd
+        assert b.items() == o.items(), rnd
+        e = np.union1d(np.setdiff1d(e, back), out)
+    for g, x in zip(b.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        assert _eq(g, x)
+    # everything erased: one border node per layer root left, plus the few nodes a
+    # layer that grew interior levels keeps
+    assert _eq(b.erase(keys, lens, mk), o.erase(keys, lens, mk))
+    assert b.items() == []
+    assert b.m.nodes(0) < full // 10 if merge else b.m.nodes(0) >= full

@@ -365,3 +365,61 @@ def test_churn_keeps_the_pool_bounded(tile):
     ok, *_, msg = g.validate()
     assert ok, msg
     assert max(used[4:]) <= 1.1 * max(used[:2]), used
+
+
+@pytest.mark.parametrize("hints", [1, 0xFFFFFFFF], ids=["write_hints", "from_root"])
+@pytest.mark.parametrize("tile", [16, 64])
+def test_erase_merges_empty_border_nodes(tile, hints):
+    """Erases that empty border nodes fold them into a neighbour and retire them: after
+    erasing a key range and a random share of the rest, the structure validates with
+    fewer than half the nodes; then rounds of mixed batches (re-inserts, erases, finds of
+    other keys) over the same ranges, and every key erased at the end.  Results equal the
+    oracle's; the slabs in use stay bounded over the rounds."""
+    rng = np.random.default_rng(81)
+    n, mk, mv = 200_000, 2, 1
+    keys, lens = W.unique_random_keys(rng, n, 1, 2)
+    keys[:, 0] %= 100_000  # ~2 keys per first slice: layers below, many border nodes
+    k = np.where(np.arange(mk)[None, :] < lens[:, None], keys, 0).astype(np.uint32)
+    _, first = np.unique(k, axis=0, return_index=True)
+    keys, lens = k[np.sort(first)], lens[np.sort(first)]
+    n = len(lens)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    g, o = S.GpuBackend(tile, pool_bytes=256 << 20), S.OracleBackend()
+    g.t.set_write_hints(hints)
+    _eq(g.insert(keys, lens, vals, None, mk, mv), o.insert(keys, lens, vals, None, mk, mv))
+    ok, _, full, _, msg = g.validate()
+    assert ok, msg
+    gone = ((keys[:, 0] >= 20_000) & (keys[:, 0] < 80_000)) | (rng.random(n) < 0.3)
+    e = np.nonzero(gone)[0]
+    _eq(g.erase(keys[e], lens[e], mk), o.erase(keys[e], lens[e], mk))
+    assert g.items() == o.items()
+    ok, _, after, _, msg = g.validate()
+    assert ok, msg
+    assert after < full // 2, (full, after)
+    used = []
+    for rnd in range(6):
+        back = rng.permutation(e)[: len(e) // 2]
+        rest = np.setdiff1d(np.arange(n), e)
+        out = rng.permutation(rest)[: len(rest) // 2]
+        probe = np.setdiff1d(rng.permutation(n)[:20_000], np.concatenate([back, out]))
+        types = np.concatenate([np.zeros(len(back), np.uint8), np.ones(len(out), np.uint8),
+                                np.full(len(probe), 2, np.uint8)])
+        idx = np.concatenate([back, out, probe])
+        p = rng.permutation(len(idx))
+        types, idx = types[p], idx[p]
+        v = vals[idx].copy()
+        vl = np.full(len(idx), mv, np.uint32)
+        for a, b in zip(g.mixed(types, keys[idx], lens[idx], v, vl, mk, mv),
+                        o.mixed(types, keys[idx], lens[idx], v.copy(), vl, mk, mv)):
+            _eq(a, b)
+        e = np.union1d(np.setdiff1d(e, back), out)
+        used.append(g.t.slabs_allocated())
+    assert g.items() == o.items()
+    ok, *_, msg = g.validate()
+    assert ok, msg
+    assert max(used[3:]) <= 1.1 * max(used[:2]), used
+    _eq(g.erase(keys, lens, mk), o.erase(keys, lens, mk))
+    assert g.items() == []
+    ok, _, left, _, msg = g.validate()
+    assert ok, msg
+    assert left < full // 10, (full, left)
