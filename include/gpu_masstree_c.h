@@ -143,6 +143,17 @@ int gmt_set_multi_mode(gmt_index* index, uint32_t mode);
  * error code since the last clear (gmt_device_error_code). */
 uint32_t gmt_device_error(gmt_index* index, int clear);
 
+/* The livelock guard's record of the operation it abandoned (GMT_DEV_STEP_LIMIT), kept by
+ * the guard whose error code was the first since the last clear: GMT_TRIP_WORDS words,
+ * trip_valid (1), phase, node, layer, layer_root, a0, a1, a2, a3, snap_hdr, the node's
+ * header when the guard fired, steps, request kind, key length, key slice of the current
+ * layer, lvl | fresh << 16.  Returns 1 and fills out when present, 0 when the guard has
+ * not fired, a negative GMT_E* code on failure; gmt_device_error(index, 1) clears it.
+ * No counterpart in the reference (it asserts); for diagnosing a stuck operation after
+ * the fact, without rerunning it. */
+#define GMT_TRIP_WORDS 16
+int gmt_guard_trip(gmt_index* index, uint32_t* out);
+
 /* Host walk of the whole tree (debug; the reference's validate(),
  * include/gpu_masstree.hpp:1421-1472).  Returns 1 if every invariant holds, 0 if not
  * (first violation in msg), negative on failure.  Any output pointer may be NULL. */

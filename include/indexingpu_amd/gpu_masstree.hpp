@@ -591,8 +591,22 @@ struct gpu_masstree {
     IGPU_HIP_TRY(hipDeviceSynchronize());
     uint32_t h = 0;
     IGPU_HIP_TRY(hipMemcpy(&h, err_, 4, hipMemcpyDeviceToHost));
-    IGPU_HIP_TRY(hipMemset(err_, 0, 4));
+    clear_error();
     return h;
+  }
+  // Clears the error word and the livelock guard's trip record.
+  void clear_error() {
+    IGPU_HIP_TRY(hipMemset(err_, 0, 4));
+    IGPU_HIP_TRY(hipMemset(err_ + indexingpu_amd::kTripOffset, 0, indexingpu_amd::kTripWords * 4));
+  }
+  // Waits for the device and copies the livelock guard's record of the operation it
+  // abandoned (indexingpu_amd::kTripWords words, layout in layout.hpp) into out; false
+  // when the guard has not fired since the last clear.
+  bool last_guard_trip(uint32_t* out) {
+    IGPU_HIP_TRY(hipDeviceSynchronize());
+    IGPU_HIP_TRY(hipMemcpy(out, err_ + indexingpu_amd::kTripOffset, indexingpu_amd::kTripWords * 4,
+                           hipMemcpyDeviceToHost));
+    return out[0] == 1;
   }
 
   // Frees the records retired so far (batch calls do this themselves).

@@ -97,11 +97,14 @@ __device__ __forceinline__ bool abort_pending(u32* err) {
   return e == kErrStepLimit || e == kErrBadIndex;
 }
 
-// Six waves per SIMD (at most 80 VGPRs; the step machine alone would take 93 and fit
-// five): the operations are latency bound, and the sixth wave measured about 10% more
-// inserts, finds, erases and mixed operations per second.
+// Five waves per SIMD (at most 96 VGPRs).  The kernel needs ~114 VGPRs to run without
+// scratch spills (4 waves); the peak is in the border-node write step (record push-down,
+// tail compare, split).  Measured on 16M-key batches: 5 waves (36 B of spills per lane)
+// run erases and mixed batches ~5% faster than 6 waves (80 VGPRs, 96 B of spills) at the
+// same insert rate; 4 waves (no spills) lose 10% on inserts, 8 waves (64 VGPRs, 336 B of
+// spills) lose 45%.
 template <u32 T, bool Validate>
-__global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6)))
+__global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5)))
 batch_kernel(tree_view tv, batch_args a) {
   wave_tile<T, Validate> c(tv.pool, tv.retire, tv.err, tv.root);
   const u32 tiles_per_block = kBlockThreads / T;

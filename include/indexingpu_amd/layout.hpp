@@ -144,4 +144,12 @@ enum error_code : u32 {
   kErrScanScratch = 6,      // scan deeper than its per-tile layer stack
 };
 
+// The livelock guard's record of the operation it abandoned, kept beside the error word
+// (words kTripOffset .. kTripOffset + kTripWords - 1 of the index's error area) by the
+// guard whose error code was the first: trip_valid (1), phase, node, layer, layer_root,
+// a0, a1, a2, a3, snap_hdr, the node's header when the guard fired, steps, request kind,
+// key length, the key slice of the current layer, lvl | fresh << 16.
+constexpr u32 kTripOffset = 16;
+constexpr u32 kTripWords = 16;
+
 }  // namespace indexingpu_amd

@@ -91,6 +91,25 @@ struct wave_tile {
   __device__ __forceinline__ void set_error(u32 code) {
     if (lane == 0) dev::set_error(err, code);
   }
+  // set_error that reports (to every lane) whether this call's code was the first
+  __device__ __forceinline__ bool set_error_first(u32 code) {
+    u32 won = 0;
+    if (lane == 0) {
+      u32 expected = 0;
+      won = __hip_atomic_compare_exchange_strong(err, &expected, code, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return bcast(won, 0) != 0;
+  }
+  __device__ __forceinline__ u32 max_op_time() const { return kMaxOpTime; }
+  // the guard's trip record, kept after the error word (layout.hpp)
+  __device__ __forceinline__ void store_trip(const u32* w) {
+    if (lane == 0)
+      for (u32 k = 0; k < kTripWords; k++) dev::coherent_store(err + kTripOffset + k, w[k]);
+  }
+  __device__ __forceinline__ u32 peek_header(u32 idx) const {
+    return idx < pool.num_slabs ? bcast(dev::coherent_load(hdr_ptr(idx)), 0) : kInvalid;
+  }
   __device__ __forceinline__ bool failed() const { return fatal; }
   // coarse device time for the per-operation time guard: the 100 MHz wall clock in
   // units of 1024 ticks (about 10 us)

@@ -103,7 +103,8 @@ struct op_state {
 // Livelock guard.  An operation may legitimately wait a long time behind a hot node
 // (thousands of tiles queue on it), so the guard is time based: after kClockCheckSteps
 // steps-or-waiting-iterations the operation starts a clock, and kMaxOpTime coarse ticks
-// (about 21 s) later it is abandoned with kErrStepLimit.  kMaxStepsPerOp bounds the
+// (about 21 s; the policy's max_op_time()) later it is abandoned with kErrStepLimit,
+// leaving a record of its state (record_trip).  kMaxStepsPerOp bounds the
 // count itself.
 constexpr u32 kClockCheckSteps = 1u << 16;
 constexpr u32 kMaxOpTime = 1u << 21;
@@ -341,6 +342,30 @@ IGPU_HD void merge_give_up(C& c, S& st) {
   op_finish(st, st.result);
 }
 
+// The guard's record of the operation it abandons (layout at kTripOffset), taken before
+// it releases anything; the node's header is read as it is now.
+template <class C, class S>
+IGPU_HD void record_trip(C& c, const request& rq, const S& st) {
+  u32 w[kTripWords];
+  w[0] = 1;
+  w[1] = st.phase;
+  w[2] = st.node;
+  w[3] = st.layer;
+  w[4] = st.layer_root;
+  w[5] = st.a0;
+  w[6] = st.a1;
+  w[7] = st.a2;
+  w[8] = st.a3;
+  w[9] = st.snap_hdr;
+  w[10] = c.peek_header(st.node);
+  w[11] = st.steps;
+  w[12] = rq.kind;
+  w[13] = rq.key_length;
+  w[14] = st.layer < rq.key_length ? c.key_slice(rq, st.layer) : 0u;
+  w[15] = (st.lvl & 0xFFFFu) | (st.fresh << 16);
+  c.store_trip(w);
+}
+
 // Returns true when the step made progress (false: retry later).
 template <bool Validate, class C, class S>
 IGPU_HD bool op_step(C& c, const request& rq, S& st) {
@@ -353,8 +378,8 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
   if (st.steps > kClockCheckSteps) {
     const u32 now = c.coarse_clock() | 1u;
     if (st.t0 == 0) st.t0 = now;
-    if (now - st.t0 > kMaxOpTime || st.steps > kMaxStepsPerOp) {
-      c.set_error(kErrStepLimit);
+    if (now - st.t0 > c.max_op_time() || st.steps > kMaxStepsPerOp) {
+      if (c.set_error_first(kErrStepLimit)) record_trip(c, rq, st);
       if (op_in_split(st)) {
         // a split waiting for its parent holds both halves (written, not yet
         // published); publish them: the right half is reachable through the left

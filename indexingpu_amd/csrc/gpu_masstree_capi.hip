@@ -302,9 +302,17 @@ uint32_t gmt_device_error(gmt_index* index, int clear) {
     uint32_t* d = B(index)->t16->err_;
     IGPU_HIP_TRY(hipDeviceSynchronize());
     IGPU_HIP_TRY(hipMemcpy(&e, d, 4, hipMemcpyDeviceToHost));
-    if (clear) IGPU_HIP_TRY(hipMemset(d, 0, 4));
+    if (clear) B(index)->t16->clear_error();
   });
   return rc == GMT_OK ? e : 0xFFFFFFFFu;
+}
+
+int gmt_guard_trip(gmt_index* index, uint32_t* out) {
+  if (!index || !out) return -bad("gmt_guard_trip: null argument");
+  bool present = false;
+  int rc = guarded([&] { present = B(index)->t16->last_guard_trip(out); });
+  if (rc != GMT_OK) return -rc;
+  return present ? 1 : 0;
 }
 
 int gmt_validate(gmt_index* index, uint64_t* entries, uint64_t* nodes, uint64_t* record_slabs,

@@ -60,6 +60,8 @@ def load_library():
     lib.gmt_mixed_batch.argtypes = [vp, P, P, u32, P, P, u32, P, P, u32, vp]
     lib.gmt_device_error.argtypes = [vp, i32]
     lib.gmt_device_error.restype = u32
+    lib.gmt_guard_trip.argtypes = [vp, vp]
+    lib.gmt_guard_trip.restype = i32
     lib.gmt_validate.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64),
                                  ctypes.POINTER(u64), ctypes.c_char_p, u32]
     lib.gmt_dump.argtypes = [vp, P, P, P, P, ctypes.POINTER(u64), ctypes.POINTER(u64)]
@@ -225,6 +227,21 @@ class GpuMasstree:
     def device_error(self, clear=True):
         """Synchronizes and returns the first device-side error code (0 = none)."""
         return int(self._lib.gmt_device_error(self._h, int(bool(clear))))
+
+    def guard_trip(self):
+        """The livelock guard's record of the operation it abandoned, as a dict, or None
+        when the guard has not fired since the last clear (device_error clears it)."""
+        import numpy as np
+        out = np.zeros(16, np.uint32)
+        rc = self._lib.gmt_guard_trip(self._h, ctypes.c_void_p(out.ctypes.data))
+        if rc < 0:
+            self._check(-rc)
+        if rc == 0:
+            return None
+        names = ("valid", "phase", "node", "layer", "layer_root", "a0", "a1", "a2", "a3",
+                 "snap_hdr", "header_now", "steps", "kind", "key_length", "key_slice",
+                 "lvl_fresh")
+        return dict(zip(names, (int(v) for v in out)))
 
     def check_device_error(self):
         e = self.device_error()

@@ -162,6 +162,21 @@ uint64_t cm_retired(void* h) { return static_cast<model*>(h)->pool.retired.size(
 void cm_set_hinted(void* h, uint32_t on) { static_cast<model*>(h)->hinted = on; }
 void cm_set_no_suffix(void* h, uint32_t on) { static_cast<model*>(h)->no_suffix = on; }
 void cm_set_erase_flags(void* h, uint32_t f) { static_cast<model*>(h)->erase_flags = f; }
+// The livelock guard's time limit in coarse ticks (10.24 us each; default about 21 s).
+void cm_set_max_op_time(void* h, uint32_t ticks) { static_cast<model*>(h)->pool.max_op_time = ticks; }
+// The guard's record of the operation it abandoned (kTripWords words, all 0 if none).
+void cm_trip(void* h, uint32_t* out) {
+  for (u32 k = 0; k < kTripWords; k++) out[k] = static_cast<model*>(h)->pool.trip[k];
+}
+uint32_t cm_root(void* h) { return static_cast<model*>(h)->root; }
+// Sets (on != 0) or clears the lock bit of node idx, as a holder that never finishes would
+// leave it; returns the header before.
+uint32_t cm_set_lock(void* h, uint32_t idx, uint32_t on) {
+  model& M = *static_cast<model*>(h);
+  u32* hp = reinterpret_cast<u32*>(M.pool.slab(idx) + kMetaSlot);
+  return on ? __atomic_fetch_or(hp, kLockBit, __ATOMIC_RELAXED)
+            : __atomic_fetch_and(hp, ~kLockBit, __ATOMIC_RELAXED);
+}
 // Allocation failure injection (0 = off): every k-th allocation, and every k-th
 // allocation of a split inserting into its parent.  Also clears the error word.
 void cm_set_fail(void* h, uint32_t any_every, uint32_t parent_every) {

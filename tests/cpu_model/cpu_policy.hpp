@@ -33,6 +33,8 @@ struct shared_pool {
   std::atomic<u32> any_allocs{0}, parent_allocs{0};
   std::mutex retire_mu;
   std::vector<u32> retired;
+  u32 max_op_time = kMaxOpTime;  // the livelock guard's time limit (coarse ticks)
+  u32 trip[kTripWords] = {};     // the guard's record of the operation it abandoned
   explicit shared_pool(u32 n) : slots((size_t)n * kSlots, 0), num_slabs(n) {}
   u64* slab(u32 idx) { return &slots[(size_t)idx * kSlots]; }
 };
@@ -49,6 +51,17 @@ struct cpu_tile {
   void set_error(u32 code) {
     u32 z = 0;
     P.error.compare_exchange_strong(z, code);
+  }
+  bool set_error_first(u32 code) {
+    u32 z = 0;
+    return P.error.compare_exchange_strong(z, code);
+  }
+  u32 max_op_time() const { return P.max_op_time; }
+  void store_trip(const u32* w) {
+    for (u32 k = 0; k < kTripWords; k++) P.trip[k] = w[k];
+  }
+  u32 peek_header(u32 idx) {
+    return idx < P.num_slabs ? __atomic_load_n(hdr_ptr(idx), __ATOMIC_RELAXED) : kInvalid;
   }
   bool fatal = false;
   bool failed() const { return fatal; }

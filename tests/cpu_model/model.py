@@ -31,6 +31,12 @@ def _load():
         lib.cm_set_fail.argtypes = [vp, u32, u32]
         lib.cm_set_no_suffix.argtypes = [vp, u32]
         lib.cm_set_erase_flags.argtypes = [vp, u32]
+        lib.cm_set_max_op_time.argtypes = [vp, u32]
+        lib.cm_trip.argtypes = [vp, u32p]
+        lib.cm_root.argtypes = [vp]
+        lib.cm_root.restype = u32
+        lib.cm_set_lock.argtypes = [vp, u32, u32]
+        lib.cm_set_lock.restype = u32
         lib.cm_unlinked.argtypes = [vp]
         lib.cm_unlinked.restype = u64
         lib.cm_nodes.argtypes = [vp, u32]
@@ -78,6 +84,24 @@ class CpuModel:
         """Request flags for every request: 4 = no merging of emptied border nodes
         (do_merge = false), 2 = emptied layers stay (do_remove_empty_root = false)."""
         self.lib.cm_set_erase_flags(self.h, flags)
+
+    def set_max_op_time(self, ticks):
+        """The livelock guard's time limit in 10.24-us ticks."""
+        self.lib.cm_set_max_op_time(self.h, ticks)
+
+    def trip(self):
+        """The guard's record of the operation it abandoned (16 words; zeros if none)."""
+        out = np.zeros(16, np.uint32)
+        self.lib.cm_trip(self.h, _p32(out))
+        return out
+
+    @property
+    def root(self):
+        return int(self.lib.cm_root(self.h))
+
+    def set_lock(self, idx, on):
+        """Sets or clears a node's lock bit; returns the header before."""
+        return int(self.lib.cm_set_lock(self.h, idx, 1 if on else 0))
 
     def set_fail(self, any_every=0, parent_every=0):
         """Injected allocation failures: every k-th allocation fails, and every k-th

@@ -375,3 +375,36 @@ def test_erase_merges_empty_border_nodes(nthreads, hinted, merge):
     assert _eq(b.erase(keys, lens, mk), o.erase(keys, lens, mk))
     assert b.items() == []
     assert b.m.nodes(0) < full // 10 if merge else b.m.nodes(0) >= full
+
+
+def test_guard_trip_record():
+    """The livelock guard, forced from the host: a border node whose lock is never
+    released (set directly, as a holder that never finishes would leave it) and a guard
+    time limit of one tick.  The insert into it is abandoned with the step-limit error and
+    the guard records the operation it abandoned: its phase, the node it waited on (with
+    that node's locked header), its layer, kind, key length and key slice.  Once the lock
+    is gone the same insert succeeds and the structure validates."""
+    from cpu_model.model import CpuModel
+    m = CpuModel(1 << 12)
+    keys = np.array([[5], [9], [7]], np.uint32)
+    lens = np.ones(3, np.uint32)
+    vals = np.array([[50], [90], [70]], np.uint32)
+    assert m.batch(0, keys[:2], 1, lens[:2], vals[:2], 1, None, n=2, ntiles=1)[2].tolist() == [1, 1]
+    assert not m.trip().any()
+    root = m.root
+    m.set_lock(root, True)
+    m.set_max_op_time(1)
+    r = m.batch(0, keys[2:], 1, lens[2:], vals[2:], 1, None, n=1, ntiles=1)[2]
+    assert r.tolist() == [0] and m.error == 2
+    t = m.trip()
+    assert t[0] == 1
+    assert t[1] in (0, 2)           # waiting to read the node (descend) or to lock it
+    assert t[2] == root and t[3] == 0 and t[4] == root
+    assert t[10] & (1 << 7)         # the node's header still carries the lock bit
+    assert t[11] > (1 << 16)        # steps taken before the clock started and ran out
+    assert t[12] == 0 and t[13] == 1 and t[14] == 7
+    m.set_lock(root, False)
+    m.set_max_op_time(1 << 21)
+    m.set_fail()  # clears the error word
+    assert m.batch(0, keys[2:], 1, lens[2:], vals[2:], 1, None, n=1, ntiles=1)[2].tolist() == [1]
+    assert m.items() == [((5,), (50,)), ((7,), (70,)), ((9,), (90,))]
