@@ -334,7 +334,7 @@ def run_batched(torch, args):
     Every request's output of every way is checked against the generator's model."""
     from indexingpu_amd import GpuMasstree
     from indexingpu_amd import device_workloads as D
-    from indexingpu_amd.autotune import Autotuner
+    from indexingpu_amd.autotune import Autotuner, FANOUTS
     n = args.batched_keys
     pool = int(args.batched_pool_gb * (1 << 30))
     log(f"[batched] generating {n} preload keys and {args.batched_count} batches ...")
@@ -386,7 +386,6 @@ def run_batched(torch, args):
     tk, tl, tv = twl.preload()
     ttree = GpuMasstree(tile_size=16, pool_bytes=pool)
     _preload(torch, ttree, tk, tl, tv, mk, mv)
-    del tk, tl, tv
     tsizes = iter(D.log_uniform_sizes(rng, 4096, 1024, 1 << 20))
     per_unit = args.batched_tune_unit
 
@@ -395,7 +394,7 @@ def run_batched(torch, args):
 
     def tune_units():
         units = []
-        for _ in range(len(space) * 2):
+        for _ in range(len(space) * 2 + 2):
             c = D.concat_batches([twl.batch(int(next(tsizes))) for _ in range(per_unit)])
             units.append(c)
         return iter(units)
@@ -419,13 +418,41 @@ def run_batched(torch, args):
     tune_table = [dict(c, mops=round(1e3 / ms / 1e6, 2) if ms > 0 else None)
                   for _, _, c, ms in tuner.report()]
     ttree.close()
-    del units
     torch.cuda.empty_cache()
+    # node fanout shapes the tree as it is built: each candidate gets a tree preloaded
+    # under it, then the same two units (the trees start identical in content)
+    fan_units = [next(units), next(units)]
+    fan_scores = []
+    for f in FANOUTS:
+        ft = GpuMasstree(tile_size=16, pool_bytes=pool)
+        ft.set_fanout(*f)
+        _preload(torch, ft, tk, tl, tv, mk, mv)
+        ft.tile_size = best["tile"]
+        ft.set_multi_grid_percent(best["grid_percent"])
+        ft.set_multi_mode(best["multi_mode"])
+        ms_total, ops = 0.0, 0
+        for c in fan_units:
+            m = len(c["types"])
+            v = c["values"].clone()
+            vl = torch.full((m,), mv, dtype=torch.int32, device="cuda")
+            r = torch.zeros(m, dtype=torch.uint8, device="cuda")
+            ms_total += timed_ms(lambda: ft.multi_batch(c["types"], c["keys"], mk, c["lens"], v, mv,
+                                                     vl, r, c["offsets"], len(c["sizes"]), m))
+            ops += m
+        ft.check_device_error()
+        ft.close()
+        torch.cuda.empty_cache()
+        fan_scores.append((f, ms_total / ops))
+        tune_table.append(dict(best, fanout=list(f), mops=round(ops / ms_total / 1e3, 2)))
+    best = dict(best, fanout=list(min(fan_scores, key=lambda x: x[1])[0]))
+    del units, fan_units, tk, tl, tv
     log(f"[batched] tuned: {best}")
 
     out = {}
     for way in ("plain", "graph", "multi_launches", "multi_kernel", "tuned"):
         tree = GpuMasstree(tile_size=16, pool_bytes=pool)
+        if way == "tuned":
+            tree.set_fanout(*best["fanout"])  # before the preload builds the tree
         pre = _preload(torch, tree, pkeys, plens, pvals, mk, mv)
         v, vl, r = fresh_outputs()
         if way == "plain":

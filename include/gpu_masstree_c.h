@@ -215,6 +215,13 @@ int gmt_set_write_hints(gmt_index* index, uint32_t min_requests);
  * No counterpart in the reference. */
 int gmt_reserve_find_scratch(gmt_index* index, uint64_t max_requests);
 
+/* Node fanout, an extension: the reference's nodes are fixed at 16 slots (14 keys per
+ * border node, 15 children per interior node; include/masstree_node_subwarp.hpp:35).
+ * Later writes split a border node once it holds border_cap entries (2..14) and an
+ * interior node once it has interior_cap children (3..15); 0 keeps the maximum.  Trees
+ * built under any setting stay readable under every other.  For the autotuner. */
+int gmt_set_fanout(gmt_index* index, uint32_t border_cap, uint32_t interior_cap);
+
 /* Tile width (lanes per operation: 16, 32 or 64) that later calls on this index use.
  * The reference fixes it per instance as the tile_size template argument
  * (include/gpu_masstree.hpp:42-46); here all widths share one node layout and one tree

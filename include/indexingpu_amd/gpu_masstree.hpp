@@ -104,7 +104,8 @@ struct gpu_masstree {
         err_(other.err_),
         barrier_(other.barrier_),
         inserted_hint_(other.inserted_hint_),
-        owner_(false) {}
+        owner_(false),
+        fanout_flags_(other.fanout_flags_) {}
   template <typename, typename, uint32_t>
   friend struct gpu_masstree;
   // Attaches to the tree of an index with another tile width: the node layout does not
@@ -117,7 +118,8 @@ struct gpu_masstree {
         err_(other.err_),
         barrier_(other.barrier_),
         inserted_hint_(other.inserted_hint_),
-        owner_(false) {}
+        owner_(false),
+        fanout_flags_(other.fanout_flags_) {}
   gpu_masstree& operator=(const gpu_masstree&) = delete;
   ~gpu_masstree() {
     if (scan_stack_) (void)hipFree(scan_stack_);  // per instance, never shared
@@ -358,6 +360,29 @@ struct gpu_masstree {
     ensure_hints(max_requests);
   }
 
+  // Node fanout (an extension; the reference's nodes are fixed at 16 slots): later writes
+  // split a border node once it holds border_cap entries (2..14) and an interior node once
+  // it has interior_cap children (3..15); 0 keeps the layout's maximum.  The nodes of one
+  // tree may have been built under different settings: readers use each node's own count.
+  // Returns false (nothing changed) for a value out of range.
+  bool set_fanout(uint32_t border_cap, uint32_t interior_cap) {
+    if ((border_cap != 0 && (border_cap < 2 || border_cap > indexingpu_amd::kBorderCap)) ||
+        (interior_cap != 0 && (interior_cap < 3 || interior_cap > indexingpu_amd::kInteriorCap)))
+      return false;
+    fanout_flags_ = (border_cap << indexingpu_amd::kReqBorderCapShift) |
+                    (interior_cap << indexingpu_amd::kReqInteriorCapShift);
+    return true;
+  }
+  uint32_t border_fanout() const {
+    const uint32_t c = (fanout_flags_ >> indexingpu_amd::kReqBorderCapShift) & 15u;
+    return c ? c : indexingpu_amd::kBorderCap;
+  }
+  uint32_t interior_fanout() const {
+    const uint32_t c = (fanout_flags_ >> indexingpu_amd::kReqInteriorCapShift) & 15u;
+    return c ? c : indexingpu_amd::kInteriorCap;
+  }
+  uint32_t fanout_flags() const { return fanout_flags_; }
+
   // Insert, update and erase batches of at least this many requests start their writes
   // at border nodes found by a read-only pass (UINT32_MAX: never).
   void set_write_hints(uint32_t min_requests) { write_hint_min_ = min_requests; }
@@ -389,6 +414,7 @@ struct gpu_masstree {
     m.a.out_values = values;
     m.a.out_value_lengths = value_lengths;
     m.a.results = results;
+    m.a.flags |= fanout_flags_;
     if (multi_mode_ == 1 && total_requests < (1ull << 32)) {
       // one batch-kernel launch per batch, each reading its range from `offsets` on the
       // device; stream order separates the batches, and the records each batch retires
@@ -545,7 +571,7 @@ struct gpu_masstree {
     c.rc = reclaimer.cache;
     indexingpu_amd::request rq{update_if_exists ? indexingpu_amd::op_upsert : indexingpu_amd::op_insert,
                                key_length, as_ptr(key), as_ptr(value), value_length, nullptr, 0,
-                               enable_suffix ? 0u : indexingpu_amd::kReqNoSuffix};
+                               (enable_suffix ? 0u : indexingpu_amd::kReqNoSuffix) | fanout_flags_};
     const bool r = run_op<true>(c, rq) != 0;
     allocator.cache = c.ac;
     reclaimer.cache = c.rc;
@@ -561,7 +587,7 @@ struct gpu_masstree {
     c.ac = allocator.cache;
     c.rc = reclaimer.cache;
     indexingpu_amd::request rq{indexingpu_amd::op_update, key_length, as_ptr(key), as_ptr(value),
-                               value_length, nullptr, 0};
+                               value_length, nullptr, 0, fanout_flags_};
     const bool r = run_op<true>(c, rq) != 0;
     allocator.cache = c.ac;
     reclaimer.cache = c.rc;
@@ -577,7 +603,7 @@ struct gpu_masstree {
     c.ac = allocator.cache;
     c.rc = reclaimer.cache;
     indexingpu_amd::request rq{indexingpu_amd::op_erase, key_length, as_ptr(key), nullptr, 0,
-                               nullptr, 0, erase_flags<do_merge, do_remove_empty_root>()};
+                               nullptr, 0, erase_flags<do_merge, do_remove_empty_root>() | fanout_flags_};
     const bool r = run_op<true>(c, rq) != 0;
     allocator.cache = c.ac;
     reclaimer.cache = c.rc;
@@ -725,8 +751,10 @@ struct gpu_masstree {
   }
 
   template <bool Validate>
-  void launch_batch(const indexingpu_amd::batch_args& a, hipStream_t stream, bool mutating) {
-    if (a.n == 0) return;
+  void launch_batch(const indexingpu_amd::batch_args& a0, hipStream_t stream, bool mutating) {
+    if (a0.n == 0) return;
+    indexingpu_amd::batch_args a = a0;
+    a.flags |= fanout_flags_;
     auto kernel = indexingpu_amd::batch_kernel<tile_size, Validate>;
     const uint32_t tiles_per_block = indexingpu_amd::kBlockThreads / tile_size;
     uint64_t want = (a.n + tiles_per_block - 1) / tiles_per_block;
@@ -996,6 +1024,7 @@ struct gpu_masstree {
   uint64_t park_words_ = 0;
   uint32_t* hints_ = nullptr;
   uint64_t hint_requests_ = 0;
+  uint32_t fanout_flags_ = 0;  // request flag bits of set_fanout (0: the layout's maximum)
   uint32_t multi_grid_percent_ = 100;
   uint32_t multi_mode_ = 1;
   uint32_t* scan_stack_ = nullptr;

@@ -5,6 +5,7 @@
 // (gpu_masstree.hpp:1432-1441), plus this layout's own B-link invariants.  Debug and
 // test use only; it never runs on the batch path.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <functional>
@@ -25,6 +26,8 @@ struct walk_stats {
   uint64_t max_depth = 0;       // deepest B-link level count in any layer
   uint64_t unlinked = 0;        // non-root nodes with no parent slot (reached by sibling links)
   uint64_t empty_borders = 0;   // non-root border nodes with no entry
+  uint64_t max_border_keys = 0;      // largest entry count of a border node
+  uint64_t max_interior_children = 0;  // largest child count of an interior node
   std::string error;            // empty when every invariant holds
 };
 
@@ -123,6 +126,7 @@ class host_walker {
         if (hdr_removed(hh)) return fail("removed node reachable: " + std::to_string(n));
         uint32_t nk = hdr_nkeys(hh);
         if (nk == 0 || nk > kInteriorCap) return fail("bad interior nkeys");
+        stats_.max_interior_children = std::max<uint64_t>(stats_.max_interior_children, nk);
         uint32_t tag = 0;
         for (uint32_t i = 0; i < kTagSlots; i++) tag |= (uint32_t)(p[i] >> 63) << i;
         if (tag != hdr_tag(hh)) return fail("interior version tag mismatch");
@@ -169,6 +173,7 @@ class host_walker {
       if (hdr_removed(hh)) return fail("removed node reachable: " + std::to_string(n));
       uint32_t nk = hdr_nkeys(hh);
       if (nk > kBorderCap) return fail("bad border nkeys");
+      stats_.max_border_keys = std::max<uint64_t>(stats_.max_border_keys, nk);
       uint32_t high = slot_lo(p[kHighSlot]);
       if (!first && high <= prev_high) return fail("border high not above left sibling's");
       if (nk == 0 && level_heads.size() > 1) stats_.empty_borders++;

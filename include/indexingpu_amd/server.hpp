@@ -409,6 +409,7 @@ class masstree_server {
       b.out_value_lengths = a.value_lengths + o0;
       b.results = reinterpret_cast<bool*>(a.results + o0);
       b.n = o1 - o0;
+      b.flags = tree_.fanout_flags();
       if (ordered_) {
         // read-only pass: finds answered, writes walked to their border nodes
         ostream = ops_stream_;

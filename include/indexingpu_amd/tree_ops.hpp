@@ -72,7 +72,7 @@ struct request {
   u32 value_length;
   u32* out_value;       // find output (max_value_length slices)
   u32 max_value_length;
-  u32 flags;  // kReqNoSuffix | kReqKeepEmptyLayers | kReqNoMerge
+  u32 flags;  // kReqNoSuffix | kReqKeepEmptyLayers | kReqNoMerge | fanout (bits 8-15)
 };
 // insert / upsert with enable_suffix = false: a new key's slices past the current layer
 // become one single-entry layer each instead of a suffix record
@@ -81,6 +81,20 @@ constexpr u32 kReqNoSuffix = 1u;
 constexpr u32 kReqKeepEmptyLayers = 2u;
 // erase<..., do_merge = false>: a border node emptied by an erase stays in place
 constexpr u32 kReqNoMerge = 4u;
+// Node fanout (an extension; the reference's nodes are fixed at 16 slots): bits 8-11 give
+// the entries at which a border node splits (2..kBorderCap) and bits 12-15 the children
+// at which an interior node splits (3..kInteriorCap); 0 means the layout's maximum.  Any
+// setting reads every tree, since readers use each node's own key count.
+constexpr u32 kReqBorderCapShift = 8;
+constexpr u32 kReqInteriorCapShift = 12;
+IGPU_HD u32 border_cap(const request& rq) {
+  const u32 c = (rq.flags >> kReqBorderCapShift) & 15u;
+  return c ? c : kBorderCap;
+}
+IGPU_HD u32 interior_cap(const request& rq) {
+  const u32 c = (rq.flags >> kReqInteriorCapShift) & 15u;
+  return c ? c : kInteriorCap;
+}
 
 template <class NodeImage>
 struct op_state {
@@ -545,7 +559,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
           return true;
         }
         const u32 ipos = c.insert_position(n, q, more);
-        if (hdr_nkeys(h) < kBorderCap) {
+        if (hdr_nkeys(h) < border_cap(rq)) {
           c.border_insert(n, ipos, q, nks, nval);
           c.write_node(st.node, n);
           c.unlock_bump(st.node);
@@ -751,7 +765,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
         // every route valid: keys up to the separator still go through the left node
         // and move right to a0; keys above it go straight to a1.
       }
-      if (hdr_nkeys(h) < kInteriorCap) {
+      if (hdr_nkeys(h) < interior_cap(rq)) {
         c.interior_insert(n, j, st.a2, lc, st.a1);
         c.write_node(st.node, n);
         c.unlock_bump(st.node);

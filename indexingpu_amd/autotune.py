@@ -11,11 +11,15 @@ The knobs (all switchable between calls on one index, same tree state):
   * grid share: percent of the resident-block capacity the batched form launches;
   * batched-form mode: one persistent launch with grid barriers, or one launch per batch.
 
-Node fanout and slab size class are not knobs here: every node is one 128-byte slab of
-16 eight-byte slots (border nodes hold up to 14 keys, interior nodes up to 15 children),
-the size one 16-lane tile reads in one coalesced access, and records use the same slab
-class.  The tuner therefore searches find tile x depth (read-only finds) and tile x
-grid share (batched form).
+  * node fanout: the entries at which later writes split a border node (2..14) and the
+    children at which they split an interior node (3..15); (0, 0) = the layout's maximum.
+    Nodes stay 128-byte slabs of 16 slots; a smaller fanout leaves room in each node
+    (fewer entries per lock, more nodes and levels).
+
+The slab size class is not a knob: every node and record slab is 128 bytes, the line one
+16-lane tile reads in one coalesced access, and the node layout (16 eight-byte slots) is
+built on it.  The tuner searches find tile x depth (read-only finds), and tile x grid share
+x fanout (mutating work).
 
 Choices are cached per (operation class, key-length class): the key-length class is the
 maximum key length rounded up to a power of two (1, 2, 4, 8, 16, 32, 64, ...), since the
@@ -37,6 +41,7 @@ FIND_DEPTHS = (1, 2, 4, 8)
 FIND_PASSES = (1, 2)
 RECORD_TILES = (0, 2, 4, 8)
 GRID_PERCENTS = (100, 50, 25)
+FANOUTS = ((0, 0), (10, 12), (7, 8))
 
 
 def length_class(max_key_length):
@@ -112,6 +117,8 @@ class Autotuner:
             self.index.set_multi_grid_percent(cfg["grid_percent"])
         if "multi_mode" in cfg:
             self.index.set_multi_mode(cfg["multi_mode"])
+        if "fanout" in cfg:
+            self.index.set_fanout(*cfg["fanout"])
 
     def apply(self, op, max_key_length):
         """Applies the cached choice for (op, key-length class); False if none."""
@@ -176,6 +183,11 @@ class Autotuner:
                 total_ops += int(box["n"])
             scored.append((dict(cfg), total_ms / max(total_ops, 1)))
         return self._pick((op, length_class(max_key_length)), scored)
+
+    def write_space(self, fanouts=FANOUTS):
+        """Candidates for tune_online over tile x grid share x node fanout."""
+        return [{"tile": t, "grid_percent": g, "fanout": tuple(f)}
+                for t, g, f in itertools.product(self.tiles, self.grid_percents, fanouts)]
 
     def report(self):
         """Rows (op, length class, config, ms per unit or per op) of every measurement."""

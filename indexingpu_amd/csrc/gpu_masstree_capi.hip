@@ -430,6 +430,15 @@ int gmt_set_find_tile(gmt_index* index, uint32_t find_tile) {
   return GMT_OK;
 }
 
+int gmt_set_fanout(gmt_index* index, uint32_t border_cap, uint32_t interior_cap) {
+  if (!index) return bad("gmt_set_fanout: null handle");
+  if ((border_cap != 0 && (border_cap < 2 || border_cap > indexingpu_amd::kBorderCap)) ||
+      (interior_cap != 0 && (interior_cap < 3 || interior_cap > indexingpu_amd::kInteriorCap)))
+    return bad("gmt_set_fanout: border_cap must be 0 or 2..14, interior_cap 0 or 3..15");
+  B(index)->on_all([&](auto& t) { t.set_fanout(border_cap, interior_cap); });
+  return GMT_OK;
+}
+
 int gmt_set_tile_size(gmt_index* index, uint32_t tile_size) {
   if (!index) return bad("gmt_set_tile_size: null handle");
   if (!valid_tile(tile_size)) return bad("gmt_set_tile_size: tile_size must be 16, 32 or 64");

@@ -60,6 +60,8 @@ def load_library():
     lib.gmt_mixed_batch.argtypes = [vp, P, P, u32, P, P, u32, P, P, u32, vp]
     lib.gmt_device_error.argtypes = [vp, i32]
     lib.gmt_device_error.restype = u32
+    lib.gmt_set_fanout.argtypes = [vp, u32, u32]
+    lib.gmt_set_fanout.restype = i32
     lib.gmt_guard_trip.argtypes = [vp, vp]
     lib.gmt_guard_trip.restype = i32
     lib.gmt_validate.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64),
@@ -227,6 +229,11 @@ class GpuMasstree:
     def device_error(self, clear=True):
         """Synchronizes and returns the first device-side error code (0 = none)."""
         return int(self._lib.gmt_device_error(self._h, int(bool(clear))))
+
+    def set_fanout(self, border_cap=0, interior_cap=0):
+        """Node fanout for later writes: border nodes split at border_cap entries (2..14),
+        interior nodes at interior_cap children (3..15); 0 = the layout's maximum."""
+        self._check(self._lib.gmt_set_fanout(self._h, border_cap, interior_cap))
 
     def guard_trip(self):
         """The livelock guard's record of the operation it abandoned, as a dict, or None

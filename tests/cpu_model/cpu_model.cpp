@@ -25,7 +25,7 @@ struct model {
   u32 root;
   u32 hinted = 0;  // writes start at hints from a read-only walk (as the GPU write passes)
   u32 no_suffix = 0;  // inserts build layers instead of suffix records (enable_suffix=false)
-  u32 erase_flags = 0;  // kReqNoMerge | kReqKeepEmptyLayers for every request
+  u32 erase_flags = 0;  // kReqNoMerge | kReqKeepEmptyLayers | fanout bits for every request
   explicit model(u32 n) : pool(n) {
     cpu_tile t(pool);
     root = t.alloc_slab();
@@ -201,14 +201,21 @@ uint64_t cm_unlinked(void* h) {
   }, nullptr);
   return w.run(M.root).unlinked;
 }
-// Reachable nodes (which = 0), border nodes (1), empty non-root border nodes (2).
+// Reachable nodes (which = 0), border nodes (1), empty non-root border nodes (2), the
+// largest border entry count (3) and interior child count (4).
 uint64_t cm_nodes(void* h, uint32_t which) {
   model& M = *static_cast<model*>(h);
   host_walker w([&](uint32_t idx) -> const uint64_t* {
     return idx < M.pool.num_slabs ? M.pool.slab(idx) : nullptr;
   }, nullptr);
   const walk_stats s = w.run(M.root);
-  return which == 0 ? s.nodes : which == 1 ? s.border_nodes : s.empty_borders;
+  switch (which) {
+    case 0: return s.nodes;
+    case 1: return s.border_nodes;
+    case 2: return s.empty_borders;
+    case 3: return s.max_border_keys;
+    default: return s.max_interior_children;
+  }
 }
 
 // Runs one batch.  nthreads == 0: interleaved simulation with ntiles tiles and the

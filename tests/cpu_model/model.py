@@ -85,6 +85,11 @@ class CpuModel:
         (do_merge = false), 2 = emptied layers stay (do_remove_empty_root = false)."""
         self.lib.cm_set_erase_flags(self.h, flags)
 
+    def set_fanout(self, border_cap=0, interior_cap=0, erase_flags=0):
+        """Node fanout for later writes (request flag bits 8-15; 0 = the maximum), with the
+        erase flags kept beside it."""
+        self.lib.cm_set_erase_flags(self.h, erase_flags | (border_cap << 8) | (interior_cap << 12))
+
     def set_max_op_time(self, ticks):
         """The livelock guard's time limit in 10.24-us ticks."""
         self.lib.cm_set_max_op_time(self.h, ticks)
@@ -119,7 +124,8 @@ class CpuModel:
         return int(self.lib.cm_retired(self.h))
 
     def nodes(self, which=0):
-        """Reachable nodes (0), border nodes (1), or empty non-root border nodes (2)."""
+        """Reachable nodes (0), border nodes (1), empty non-root border nodes (2), the
+        largest border entry count (3) or interior child count (4)."""
         return int(self.lib.cm_nodes(self.h, which))
 
     @property

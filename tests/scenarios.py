@@ -62,11 +62,13 @@ class OracleBackend:
 class GpuBackend:
     name = "gpu"
 
-    def __init__(self, tile_size=16, pool_bytes=64 << 20):
+    def __init__(self, tile_size=16, pool_bytes=64 << 20, fanout=(0, 0)):
         import torch
         from indexingpu_amd import GpuMasstree
         self.torch = torch
         self.t = GpuMasstree(tile_size=tile_size, pool_bytes=pool_bytes)
+        if fanout != (0, 0):
+            self.t.set_fanout(*fanout)
 
     def _d(self, a, dtype=np.uint32):
         torch = self.torch
@@ -263,10 +265,12 @@ class CpuModelBackend:
     name = "cpu_model"
 
     def __init__(self, nthreads=0, ntiles=32, seed=0, hinted=False, allow_error=False,
-                 no_suffix=False):
+                 no_suffix=False, fanout=(0, 0)):
         from cpu_model.model import CpuModel
         self.m = CpuModel(1 << 21)
         self.m.set_no_suffix(no_suffix)
+        if fanout != (0, 0):
+            self.m.set_fanout(*fanout)
         self.allow_error = allow_error
         self.m.set_hinted(hinted)
         self.nthreads, self.ntiles, self.seed = nthreads, ntiles, seed

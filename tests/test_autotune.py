@@ -28,6 +28,9 @@ class FakeIndex:
     def set_multi_grid_percent(self, p):
         self.grid = p
 
+    def set_fanout(self, b, i):
+        self.fanout = (b, i)
+
 
 class FakeClock:
     """Milliseconds = cost(knobs of the index at the time fn runs)."""
@@ -105,3 +108,25 @@ def test_apply_restores_cached_choice():
     assert tu.apply("find", 3)
     assert (idx.passes, idx.ftile, idx.depth) == (2, 1, 1)
     assert not tu.apply("mixed", 3)
+
+
+def test_tune_online_over_fanout():
+    """The write space spans tile x grid share x node fanout; the fanout of the winner is
+    applied to the index."""
+    idx = FakeIndex()
+    idx.fanout = (0, 0)
+
+    def run_unit(cfg):
+        assert idx.fanout == cfg["fanout"]
+        return 1000
+
+    def cost(t, d, g, f):
+        b = idx.fanout[0] or 14
+        return 1000 * (1.0 + abs(t - 32) / 64 + abs(g - 100) / 100 + abs(b - 10) / 10)
+
+    tu = Autotuner(idx, trials=1, timer=FakeClock(idx, cost))
+    space = tu.write_space()
+    assert len(space) == 3 * 3 * 3
+    cfg = tu.tune_online("mixed", run_unit, max_key_length=4, space=space)
+    assert cfg == {"tile": 32, "grid_percent": 100, "fanout": (10, 12)}
+    assert idx.fanout == (10, 12) and idx.tile_size == 32

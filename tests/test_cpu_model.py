@@ -408,3 +408,41 @@ def test_guard_trip_record():
     m.set_fail()  # clears the error word
     assert m.batch(0, keys[2:], 1, lens[2:], vals[2:], 1, None, n=1, ntiles=1)[2].tolist() == [1]
     assert m.items() == [((5,), (50,)), ((7,), (70,)), ((9,), (90,))]
+
+
+@pytest.mark.parametrize("fanout", [(2, 3), (3, 4), (6, 5)], ids=["b2i3", "b3i4", "b6i5"])
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_small_fanout(nthreads, fanout):
+    """Node fanout below the layout's maximum (border nodes split at 2, 3 or 6 entries,
+    interior nodes at 3, 4 or 5 children): every insert of a growing tree splits and most
+    splits go up several levels, while mixed batches, hinted writes and erases that empty
+    and merge nodes run over the same keys.  Results equal the oracle's, the walk finds the
+    structure valid with no node above the fanout, and the tree is much deeper than at full
+    fanout.  The fanout then changes back to the maximum on the same tree."""
+    wl = W.MixedWorkload(6000, key_len=3, seed=17)
+    wl.keys[:, 0] %= 300
+    wl.hashes = W.key_hash64(wl.keys, wl.lens)
+    mk, mv = 3, 2
+    b = S.CpuModelBackend(nthreads=nthreads, hinted=True, fanout=fanout)
+    o = S.OracleBackend()
+    assert np.array_equal(b.insert(wl.keys, wl.lens, wl.values, None, mk, mv),
+                          o.insert(wl.keys, wl.lens, wl.values, None, mk, mv))
+    assert b.items() == o.items()
+    small = b.m.nodes(0)
+    assert b.m.nodes(3) <= fanout[0] and b.m.nodes(4) <= fanout[1]
+    for rnd in range(4):
+        if rnd == 2:
+            b.m.set_fanout(0, 0)  # back to full nodes: later splits fill them up
+        bt = wl.batch(1500)
+        vl = np.full(len(bt["types"]), mv, np.uint32)
+        got = b.mixed(bt["types"], bt["keys"], bt["lens"], bt["values"], vl, mk, mv)
+        exp = o.mixed(bt["types"], bt["keys"], bt["lens"], bt["values"], vl, mk, mv)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e), rnd
+        assert b.items() == o.items(), rnd
+    full = S.CpuModelBackend(nthreads=0)
+    full.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
+    assert small > 2 * full.m.nodes(0)
+    e = np.arange(0, len(wl.lens), 3)
+    assert np.array_equal(b.erase(wl.keys[e], wl.lens[e], mk), o.erase(wl.keys[e], wl.lens[e], mk))
+    assert b.items() == o.items()

@@ -423,3 +423,44 @@ def test_erase_merges_empty_border_nodes(tile, hints):
     ok, _, left, _, msg = g.validate()
     assert ok, msg
     assert left < full // 10, (full, left)
+
+
+@pytest.mark.parametrize("fanout", [(3, 4), (8, 6)], ids=["b3i4", "b8i6"])
+@pytest.mark.parametrize("tile", [16, 64])
+def test_small_fanout(tile, fanout):
+    """Node fanout below the layout's maximum (the autotuner's fanout knob): inserts of
+    var-len keys into an empty tree split nearly every border node and carry splits up
+    several interior levels; then mixed batches (split into a read-only pass and the
+    writes), scans, erases that empty and merge nodes, and finds of every key.  Results
+    equal the oracle's, the structure validates, and the fanout then returns to the
+    maximum on the same tree for more mixed batches."""
+    wl = W.MixedWorkload(100_000, key_len=4, seed=19)
+    mk, mv = 4, 2
+    g, o = S.GpuBackend(tile, pool_bytes=256 << 20, fanout=fanout), S.OracleBackend()
+    _eq(g.insert(wl.keys, wl.lens, wl.values, None, mk, mv),
+        o.insert(wl.keys, wl.lens, wl.values, None, mk, mv))
+    ok, _, nodes_small, _, msg = g.validate()
+    assert ok, msg
+    for rnd in range(4):
+        if rnd == 2:
+            g.t.set_fanout(0, 0)
+        bt = wl.batch(30_000)
+        vl = np.full(len(bt["types"]), mv, np.uint32)
+        for a, b in zip(g.mixed(bt["types"], bt["keys"], bt["lens"], bt["values"], vl, mk, mv),
+                        o.mixed(bt["types"], bt["keys"], bt["lens"], bt["values"], vl, mk, mv)):
+            _eq(a, b)
+    sc = wl.scans(2048, max_scan=100)
+    lens = np.full(len(sc["limits"]), mk, np.uint32)
+    for a, b in zip(g.scan(sc["lower"], lens, mk, 100, maxV=mv, limits=sc["limits"]),
+                    o.scan(sc["lower"], lens, mk, 100, maxV=mv, limits=sc["limits"])):
+        _eq(a, b)
+    e = np.arange(0, len(wl.lens), 2)
+    _eq(g.erase(wl.keys[e], wl.lens[e], mk), o.erase(wl.keys[e], wl.lens[e], mk))
+    for a, b in zip(g.find(wl.keys, wl.lens, mk, mv), o.find(wl.keys, wl.lens, mk, mv)):
+        _eq(a, b)
+    assert g.items() == o.items()
+    ok, *_, msg = g.validate()
+    assert ok, msg
+    full = S.GpuBackend(tile, pool_bytes=256 << 20)
+    full.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
+    assert nodes_small > 2 * full.validate()[2]
