@@ -125,6 +125,7 @@ struct gpu_masstree {
     if (scan_stack_) (void)hipFree(scan_stack_);  // per instance, never shared
     if (park_) (void)hipFree(park_);
     if (hints_) (void)hipFree(hints_);
+    if (shortcut_) (void)hipFree(shortcut_);
     if (owner_) {
       if (err_) (void)hipFree(err_);
       if (barrier_) (void)hipFree(barrier_);
@@ -148,6 +149,7 @@ struct gpu_masstree {
     if (concurrent) {
       launch_batch<true>(a, stream, false);
     } else {
+      refresh_shortcut(stream);
       switch (find_tile_) {
         case 1: launch_narrow<1>(a, stream); break;
         case 2: launch_narrow<2>(a, stream); break;
@@ -346,6 +348,7 @@ struct gpu_masstree {
       // that of one launch running all requests together.
       ensure_hints(num_requests);
       a.hints = hints_;
+      refresh_shortcut(stream);
       launch_finds_pass(a, stream);
       a.skip_finds = 1;
     }
@@ -383,6 +386,23 @@ struct gpu_masstree {
   }
   uint32_t fanout_flags() const { return fanout_flags_; }
 
+  // Read-only descents of large trees (find, the read-only pass of mixed batches, the
+  // start hints of write batches) start at a layer-0 shortcut table (refresh_shortcut)
+  // instead of the root: on (the default) or off (the table is dropped).  An extension;
+  // results do not depend on it.
+  void set_find_shortcut(bool on) {
+    shortcut_on_ = on;
+    if (!on && shortcut_) {
+      IGPU_HIP_TRY(hipDeviceSynchronize());
+      IGPU_HIP_TRY(hipFree(shortcut_));
+      shortcut_ = nullptr;
+      shortcut_bits_ = shortcut_capacity_bits_ = 0;
+      shortcut_size_ = 0;
+    }
+  }
+  bool find_shortcut() const { return shortcut_on_; }
+  uint32_t find_shortcut_bits() const { return shortcut_ ? shortcut_bits_ : 0u; }
+
   // Insert, update and erase batches of at least this many requests start their writes
   // at border nodes found by a read-only pass (UINT32_MAX: never).
   void set_write_hints(uint32_t min_requests) { write_hint_min_ = min_requests; }
@@ -416,6 +436,7 @@ struct gpu_masstree {
     m.a.results = results;
     m.a.flags |= fanout_flags_;
     if (multi_mode_ == 1 && total_requests < (1ull << 32)) {
+      refresh_shortcut(stream);
       // one batch-kernel launch per batch, each reading its range from `offsets` on the
       // device; stream order separates the batches, and the records each batch retires
       // are drained right after it
@@ -727,7 +748,40 @@ struct gpu_masstree {
   }
 
   indexingpu_amd::tree_view view() const {
-    return indexingpu_amd::tree_view{allocator_, reclaimer_, err_, root_index_};
+    return indexingpu_amd::tree_view{allocator_, reclaimer_, err_, root_index_, shortcut_,
+                                     shortcut_ ? 32u - shortcut_bits_ : 0u};
+  }
+
+  // The layer-0 shortcut table of the read-only descents (tree_view::shortcut): built
+  // once the tree holds kShortcutMinKeys keys (by the insert count), with one entry per
+  // ~64 keys (2^12..2^24 entries), and rebuilt when the count has doubled since.  A table
+  // left from before later writes is still correct (see tree_view), only less precise.
+  // Not built under graph capture (no allocation there); an existing table is used.
+  static constexpr uint64_t kShortcutMinKeys = 1ull << 20;
+  void refresh_shortcut(hipStream_t stream) {
+    if (!shortcut_on_) return;
+    const uint64_t size = *inserted_hint_;
+    if (size < kShortcutMinKeys) return;
+    uint32_t bits = 12;
+    while (bits < 24 && (1ull << bits) * 64 < size) bits++;
+    if (shortcut_ && size < 2 * shortcut_size_) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    IGPU_HIP_TRY(hipStreamIsCapturing(stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return;
+    if (bits > shortcut_capacity_bits_) {
+      IGPU_HIP_TRY(hipDeviceSynchronize());
+      if (shortcut_) IGPU_HIP_TRY(hipFree(shortcut_));
+      shortcut_ = nullptr;
+      IGPU_HIP_TRY(hipMalloc(&shortcut_, (4ull << bits)));
+      shortcut_capacity_bits_ = bits;
+    }
+    const uint32_t entries = 1u << bits;
+    hipLaunchKernelGGL(indexingpu_amd::shortcut_build_kernel, dim3((entries + 255) / 256),
+                       dim3(256), 0, stream, indexingpu_amd::tree_view{allocator_, reclaimer_,
+                                                                       err_, root_index_},
+                       shortcut_, entries, 32u - bits);
+    shortcut_bits_ = bits;
+    shortcut_size_ = size;
   }
 
   // Keys of more than 65,536 slices are refused (the device error word reports
@@ -837,6 +891,7 @@ struct gpu_masstree {
       return;
     }
     ensure_hints(std::min<uint64_t>(n, kHintChunk));
+    refresh_shortcut(stream);
     for (uint64_t off = 0; off < n; off += kHintChunk) {
       indexingpu_amd::batch_args c = a;
       c.n = std::min<uint64_t>(kHintChunk, n - off);
@@ -1025,6 +1080,11 @@ struct gpu_masstree {
   uint32_t* hints_ = nullptr;
   uint64_t hint_requests_ = 0;
   uint32_t fanout_flags_ = 0;  // request flag bits of set_fanout (0: the layout's maximum)
+  uint32_t* shortcut_ = nullptr;     // layer-0 shortcut table (refresh_shortcut)
+  uint32_t shortcut_bits_ = 0;       // log2 of its entries
+  uint32_t shortcut_capacity_bits_ = 0;
+  uint64_t shortcut_size_ = 0;       // insert count when it was built
+  bool shortcut_on_ = true;
   uint32_t multi_grid_percent_ = 100;
   uint32_t multi_mode_ = 1;
   uint32_t* scan_stack_ = nullptr;

@@ -29,6 +29,13 @@ struct tree_view {
   retire_view retire;
   u32* err;
   u32 root;
+  // Optional layer-0 shortcut table for read-only descents (null: none): entry i is an
+  // interior node of level 1 (or the tree root) whose key range starts at or below the
+  // first slices of prefix i = slice >> shortcut_shift.  Interior nodes of layer 0 are
+  // never freed, never change level (only the root grows, in place) and their low bounds
+  // never rise, so an entry stays a valid B-link starting point after any later write.
+  const u32* shortcut;
+  u32 shortcut_shift;
 };
 
 // Arguments of one batch.  `types` selects per-request kinds (mixed batches); when it
@@ -367,6 +374,38 @@ multi_batch_kernel(tree_view tv, multi_args m) {
     if (__ballot(!progressed) == __ballot(true)) __builtin_amdgcn_s_sleep(1);
   }
   c.flush();
+}
+
+// Builds the layer-0 shortcut table (tree_view::shortcut): entry i is the level-1 node
+// (the root when the tree has fewer levels) reached from the root by the smallest first
+// slice of prefix i.  Runs with no writer in the launch; the walk moves right past bounds
+// below the slice as any descent does.
+__global__ void shortcut_build_kernel(tree_view tv, u32* table, u32 entries, u32 shift) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= entries) return;
+  const u32 q = i << shift;
+  u32 node = tv.root;
+  for (u32 guard = 0; guard < 4096; guard++) {
+    const u64* p = tv.pool.pool + (u64)node * kSlots;
+    const u32 h = slot_lo(p[kMetaSlot]);
+    if (hdr_border(h) || hdr_level(h) <= 1) break;
+    const u32 nk = hdr_nkeys(h);
+    u32 next = kInvalid;
+    for (u32 j = 0; j < nk && j < kInteriorCap; j++) {
+      const u64 e = p[j];
+      if (q <= slot_lo(e)) {
+        next = child_of(e);
+        break;
+      }
+    }
+    if (next == kInvalid) next = slot_hi(p[kMetaSlot]);  // right of every bound: sibling
+    if (next >= tv.pool.num_slabs) {
+      node = tv.root;  // not expected in a quiescent tree: fall back to the root
+      break;
+    }
+    node = next;
+  }
+  table[i] = node;
 }
 
 __global__ void zero_kernel(u32* p, u32 n) {

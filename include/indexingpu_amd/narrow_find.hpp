@@ -138,10 +138,10 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_find_kernel(tree_view tv
         const u32 s = c.j + u * T;
         kc[k][u] = s < L ? key[s] : 0u;
       }
-      node[k] = tv.root;
       layer[k] = 0;
       phase[k] = kFindDescend;
       q[k] = slice(k, 0);
+      node[k] = tv.shortcut ? tv.shortcut[q[k] >> tv.shortcut_shift] : tv.root;
       return;
     }
   };
@@ -454,7 +454,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       lroot[k] = tv.root;
       par[k] = kInvalid;
     }
-    node[k] = tv.root;
+    node[k] = tv.shortcut ? tv.shortcut[q0 >> tv.shortcut_shift] : tv.root;
     layer[k] = 0;
     q_stale[k] = false;
     active[k] = idx[k] < hi;

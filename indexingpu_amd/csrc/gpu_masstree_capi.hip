@@ -430,6 +430,18 @@ int gmt_set_find_tile(gmt_index* index, uint32_t find_tile) {
   return GMT_OK;
 }
 
+int gmt_set_find_shortcut(gmt_index* index, int on) {
+  if (!index) return bad("gmt_set_find_shortcut: null handle");
+  return guarded([&] { B(index)->on_all([&](auto& t) { t.set_find_shortcut(on != 0); }); });
+}
+
+uint32_t gmt_find_shortcut_bits(gmt_index* index) {
+  if (!index) return 0;
+  uint32_t b = 0;
+  guarded([&] { B(index)->on_all([&](auto& t) { b = std::max(b, t.find_shortcut_bits()); }); });
+  return b;
+}
+
 int gmt_set_fanout(gmt_index* index, uint32_t border_cap, uint32_t interior_cap) {
   if (!index) return bad("gmt_set_fanout: null handle");
   if ((border_cap != 0 && (border_cap < 2 || border_cap > indexingpu_amd::kBorderCap)) ||

@@ -60,6 +60,10 @@ def load_library():
     lib.gmt_mixed_batch.argtypes = [vp, P, P, u32, P, P, u32, P, P, u32, vp]
     lib.gmt_device_error.argtypes = [vp, i32]
     lib.gmt_device_error.restype = u32
+    lib.gmt_set_find_shortcut.argtypes = [vp, i32]
+    lib.gmt_set_find_shortcut.restype = i32
+    lib.gmt_find_shortcut_bits.argtypes = [vp]
+    lib.gmt_find_shortcut_bits.restype = u32
     lib.gmt_set_fanout.argtypes = [vp, u32, u32]
     lib.gmt_set_fanout.restype = i32
     lib.gmt_guard_trip.argtypes = [vp, vp]
@@ -229,6 +233,15 @@ class GpuMasstree:
     def device_error(self, clear=True):
         """Synchronizes and returns the first device-side error code (0 = none)."""
         return int(self._lib.gmt_device_error(self._h, int(bool(clear))))
+
+    def set_find_shortcut(self, on):
+        """Read-only descents of large trees start at the layer-0 shortcut table (True, the
+        default) or at the root (False)."""
+        self._check(self._lib.gmt_set_find_shortcut(self._h, 1 if on else 0))
+
+    def find_shortcut_bits(self):
+        """log2 of the shortcut table's entries (0: no table)."""
+        return int(self._lib.gmt_find_shortcut_bits(self._h))
 
     def set_fanout(self, border_cap=0, interior_cap=0):
         """Node fanout for later writes: border nodes split at border_cap entries (2..14),

@@ -464,3 +464,52 @@ def test_small_fanout(tile, fanout):
     full = S.GpuBackend(tile, pool_bytes=256 << 20)
     full.insert(wl.keys, wl.lens, wl.values, None, mk, mv)
     assert nodes_small > 2 * full.validate()[2]
+
+
+@pytest.mark.parametrize("find_tile", [2, 8])
+def test_find_shortcut_table(find_tile):
+    """Read-only descents of a tree of more than 2^20 keys start at the layer-0 shortcut
+    table (level-1 nodes by first-slice prefix).  Finds of present and absent keys, a split
+    mixed batch (its read-only pass and write hints use the table), then erases that merge
+    nodes and inserts that split them while the table stays as built (stale), and finds
+    again: every result equals the oracle's, and equals the same calls with the table off."""
+    rng = np.random.default_rng(23)
+    n, mk, mv = 1_300_000, 4, 2
+    keys, lens = W.unique_random_keys(rng, n, 1, mk)
+    keys = np.where(np.arange(mk)[None, :] < lens[:, None], keys, 0).astype(np.uint32)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    g, o = S.GpuBackend(16, pool_bytes=1 << 30), S.OracleBackend()
+    g.t.set_find_tile(find_tile)
+    base = n - 200_000
+    _eq(g.insert(keys[:base], lens[:base], vals[:base], None, mk, mv),
+        o.insert(keys[:base], lens[:base], vals[:base], None, mk, mv))
+    probe = np.concatenate([rng.permutation(n)[:300_000]])
+    for a, b in zip(g.find(keys[probe], lens[probe], mk, mv), o.find(keys[probe], lens[probe], mk, mv)):
+        _eq(a, b)
+    assert g.t.find_shortcut_bits() > 0
+    # a split mixed batch: finds of present keys, inserts of the rest, erases of others
+    ins = np.arange(base, n)
+    era = rng.permutation(base)[:150_000]
+    fnd = np.setdiff1d(rng.permutation(base)[:100_000], era)
+    types = np.concatenate([np.zeros(len(ins), np.uint8), np.ones(len(era), np.uint8),
+                            np.full(len(fnd), 2, np.uint8)])
+    idx = np.concatenate([ins, era, fnd])
+    p = rng.permutation(len(idx))
+    types, idx = types[p], idx[p]
+    v = vals[idx].copy()
+    vl = np.full(len(idx), mv, np.uint32)
+    for a, b in zip(g.mixed(types, keys[idx], lens[idx], v, vl, mk, mv),
+                    o.mixed(types, keys[idx], lens[idx], v.copy(), vl, mk, mv)):
+        _eq(a, b)
+    # more erases and inserts, then finds through the table as built before them
+    _eq(g.erase(keys[era[:50_000]], lens[era[:50_000]], mk), o.erase(keys[era[:50_000]], lens[era[:50_000]], mk))
+    _eq(g.insert(keys[era], lens[era], vals[era], None, mk, mv), o.insert(keys[era], lens[era], vals[era], None, mk, mv))
+    with_table = g.find(keys, lens, mk, mv)
+    for a, b in zip(with_table, o.find(keys, lens, mk, mv)):
+        _eq(a, b)
+    g.t.set_find_shortcut(False)
+    assert g.t.find_shortcut_bits() == 0
+    for a, b in zip(g.find(keys, lens, mk, mv), with_table):
+        _eq(a, b)
+    ok, *_, msg = g.validate()
+    assert ok, msg
