@@ -216,11 +216,12 @@ int gmt_set_write_hints(gmt_index* index, uint32_t min_requests);
 int gmt_reserve_find_scratch(gmt_index* index, uint64_t max_requests);
 
 /* Read-only descents (find, the read-only pass of mixed batches, the start hints of write
- * batches) of a tree of at least 2^20 inserted keys start at a layer-0 shortcut table (one
- * entry per ~64 keys: the level-1 node holding the smallest key of each prefix of the
- * first slice) instead of the root: on (the default) or off.  An extension; results do not
- * depend on it.  The table is rebuilt when the insert count has doubled; one left from
- * before later writes is still correct. */
+ * batches) of a tree of at least 2^20 inserted keys whose root is at level 3 or above
+ * start at a layer-0 shortcut table (about one 4-byte entry per key, at most 2^26: the
+ * level-1 node holding the smallest key of each prefix of the first slice) instead of the
+ * root: on (the default) or off.  An extension; results do not depend on it.  The table
+ * is rebuilt when the insert count has doubled; one left from before later writes is
+ * still correct. */
 int gmt_set_find_shortcut(gmt_index* index, int on);
 /* log2 of the entries of the largest shortcut table the index holds (0: none). */
 uint32_t gmt_find_shortcut_bits(gmt_index* index);

@@ -753,21 +753,33 @@ struct gpu_masstree {
   }
 
   // The layer-0 shortcut table of the read-only descents (tree_view::shortcut): built
-  // once the tree holds kShortcutMinKeys keys (by the insert count), with one entry per
-  // ~64 keys (2^12..2^24 entries), and rebuilt when the count has doubled since.  A table
-  // left from before later writes is still correct (see tree_view), only less precise.
-  // Not built under graph capture (no allocation there); an existing table is used.
+  // once the tree holds kShortcutMinKeys keys (by the insert count) and its root is at
+  // level 3 or above, with about one entry per key (2^20..2^26 entries, 4 B each), and
+  // rebuilt when the count has doubled since.  Measured at 64M var-len keys (two-pass
+  // finds, MI355X): a batch takes 9.1 ms from the root, 8.3 ms with one entry per 64 keys,
+  // 7.3 with one per 4 and 6.9 with one per key (each prefix then starts at its own
+  // level-1 node with no move-right).  A table left from before later writes is still
+  // correct (see tree_view), only less precise.  Not built under graph capture (no
+  // allocation there); an existing table is used.
   static constexpr uint64_t kShortcutMinKeys = 1ull << 20;
+  static constexpr uint32_t kShortcutMaxBits = 26;
   void refresh_shortcut(hipStream_t stream) {
     if (!shortcut_on_) return;
     const uint64_t size = *inserted_hint_;
-    if (size < kShortcutMinKeys) return;
-    uint32_t bits = 12;
-    while (bits < 24 && (1ull << bits) * 64 < size) bits++;
-    if (shortcut_ && size < 2 * shortcut_size_) return;
+    if (size < kShortcutMinKeys || (shortcut_size_ && size < 2 * shortcut_size_)) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     IGPU_HIP_TRY(hipStreamIsCapturing(stream, &cs));
     if (cs != hipStreamCaptureStatusNone) return;
+    shortcut_size_ = size;  // decided for this size, table or not
+    // a root below level 3 leaves at most one interior level to skip
+    uint64_t meta = 0;
+    IGPU_HIP_TRY(hipStreamSynchronize(stream));
+    IGPU_HIP_TRY(hipMemcpy(&meta, allocator_.pool + (uint64_t)root_index_ * indexingpu_amd::kSlots +
+                                      indexingpu_amd::kMetaSlot, 8, hipMemcpyDeviceToHost));
+    const uint32_t rh = indexingpu_amd::slot_lo(meta);
+    if (indexingpu_amd::hdr_border(rh) || indexingpu_amd::hdr_level(rh) < 3) return;
+    uint32_t bits = 20;
+    while (bits < kShortcutMaxBits && (1ull << bits) < size) bits++;
     if (bits > shortcut_capacity_bits_) {
       IGPU_HIP_TRY(hipDeviceSynchronize());
       if (shortcut_) IGPU_HIP_TRY(hipFree(shortcut_));
@@ -781,7 +793,6 @@ struct gpu_masstree {
                                                                        err_, root_index_},
                        shortcut_, entries, 32u - bits);
     shortcut_bits_ = bits;
-    shortcut_size_ = size;
   }
 
   // Keys of more than 65,536 slices are refused (the device error word reports
