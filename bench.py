@@ -608,7 +608,7 @@ def main():
     ap.add_argument("--served-keys", type=int, default=1 << 29)
     ap.add_argument("--served-pool-gb", type=float, default=112.0)
     ap.add_argument("--served-rates",
-                    default="1000,3000,10000,30000,100000,300000,600000,800000,1000000,2000000")
+                    default="1000,3000,10000,30000,100000,300000,600000,800000,900000,1000000,2000000")
     ap.add_argument("--served-seconds", type=float, default=0.25)
     ap.add_argument("--served-max-requests", type=int, default=1_000_000)
     ap.add_argument("--served-wave-ops", type=int, default=1 << 18)

@@ -708,6 +708,15 @@ struct gpu_masstree {
   }
   std::size_t num_nodes_used() { return allocator_.num_allocated_slabs(); }
 
+  // The layer-0 shortcut table for read-only descents launched on `stream`, refreshed
+  // first (which may synchronize the stream): {table, shift}, table null when there is
+  // none.  For callers that launch the narrow descent kernels themselves (the served
+  // form); a table stays correct under later writes.
+  std::pair<const uint32_t*, uint32_t> find_shortcut_table(hipStream_t stream) {
+    refresh_shortcut(stream);
+    return {shortcut_, shortcut_ ? 32u - shortcut_bits_ : 0u};
+  }
+
   size_type root_index_ = 0;
   device_allocator_instance_type allocator_;
   device_reclaimer_instance_type reclaimer_;
@@ -754,14 +763,14 @@ struct gpu_masstree {
 
   // The layer-0 shortcut table of the read-only descents (tree_view::shortcut): built
   // once the tree holds kShortcutMinKeys keys (by the insert count) and its root is at
-  // level 3 or above, with about one entry per key (2^20..2^26 entries, 4 B each), and
+  // level 3 or above, with about one entry per key (2^18..2^26 entries, 4 B each), and
   // rebuilt when the count has doubled since.  Measured at 64M var-len keys (two-pass
   // finds, MI355X): a batch takes 9.1 ms from the root, 8.3 ms with one entry per 64 keys,
   // 7.3 with one per 4 and 6.9 with one per key (each prefix then starts at its own
   // level-1 node with no move-right).  A table left from before later writes is still
   // correct (see tree_view), only less precise.  Not built under graph capture (no
   // allocation there); an existing table is used.
-  static constexpr uint64_t kShortcutMinKeys = 1ull << 20;
+  static constexpr uint64_t kShortcutMinKeys = 1ull << 18;
   static constexpr uint32_t kShortcutMaxBits = 26;
   void refresh_shortcut(hipStream_t stream) {
     if (!shortcut_on_) return;
@@ -778,7 +787,7 @@ struct gpu_masstree {
                                       indexingpu_amd::kMetaSlot, 8, hipMemcpyDeviceToHost));
     const uint32_t rh = indexingpu_amd::slot_lo(meta);
     if (indexingpu_amd::hdr_border(rh) || indexingpu_amd::hdr_level(rh) < 3) return;
-    uint32_t bits = 20;
+    uint32_t bits = 18;
     while (bits < kShortcutMaxBits && (1ull << bits) < size) bits++;
     if (bits > shortcut_capacity_bits_) {
       IGPU_HIP_TRY(hipDeviceSynchronize());
@@ -879,6 +888,7 @@ struct gpu_masstree {
       indexingpu_amd::batch_args s = a;
       s.perm_lo = done;
       s.n = stage;
+      *inserted_hint_ = size;  // the stage's hint descent may use a shortcut table
       launch_hinted(s, stream);
       done += stage;
       size += stage;

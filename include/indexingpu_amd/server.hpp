@@ -160,6 +160,13 @@ class masstree_server {
     }
     IGPU_HIP_TRY(hipMalloc(&census_dev_, 8));
     IGPU_HIP_TRY(hipHostMalloc(&census_host_, 8, hipHostMallocDefault));
+    // read-only passes of ordered waves start at the tree's layer-0 shortcut table (built
+    // here if due; it stays correct while the waves write)
+    if (ordered_) {
+      IGPU_HIP_TRY(hipDeviceSynchronize());
+      shortcut_ = tree_.find_shortcut_table(ops_stream_);
+      IGPU_HIP_TRY(hipStreamSynchronize(ops_stream_));
+    }
   }
   ~masstree_server() {
     (void)hipDeviceSynchronize();
@@ -363,7 +370,7 @@ class masstree_server {
 
   tree_view view_with(int list) const {
     return tree_view{tree_.allocator_, lists_[list].rec->get_device_instance(), tree_.err_,
-                     tree_.root_index_};
+                     tree_.root_index_, shortcut_.first, shortcut_.second};
   }
 
   void launch(const serve_arena& a, const serve_request* req, const wave& w, size_t s,
@@ -547,6 +554,7 @@ class masstree_server {
   }
 
   Tree& tree_;
+  std::pair<const uint32_t*, uint32_t> shortcut_{nullptr, 0u};  // layer-0 shortcut table
   serve_config cfg_;
   static constexpr uint32_t kOrderedTile = 2;  // lanes per lookup of the read-only pass
   bool ordered_ = false;
