@@ -468,7 +468,7 @@ def test_small_fanout(tile, fanout):
 
 @pytest.mark.parametrize("find_tile", [2, 8])
 def test_find_shortcut_table(find_tile):
-    """Read-only descents of a tree of more than 2^20 keys start at the layer-0 shortcut
+    """Read-only descents of a tree of more than 2^18 keys start at the layer-0 shortcut
     table (level-1 nodes by first-slice prefix).  Finds of present and absent keys, a split
     mixed batch (its read-only pass and write hints use the table), then erases that merge
     nodes and inserts that split them while the table stays as built (stale), and finds
