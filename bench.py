@@ -631,9 +631,9 @@ def main():
     args.no_cpu_baseline = args.no_cpu_baseline or "cpu" in skip
     args.no_ceiling = args.no_ceiling or "ceiling" in skip
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: every box in this pool has one GPU, so "
-              f"--gpus {args.gpus} is not supported (the index shards by key range across "
-              f"GPUs; see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (a "
+              f"multi-GPU index would shard by key range on the first slice; not built).",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

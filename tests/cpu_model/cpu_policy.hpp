@@ -322,7 +322,7 @@ struct cpu_tile {
   }
 
   // ---------------- slabs ----------------
-  bool in_parent = false;  // set by the driver while the operation is in ph_parent_lock
+  bool in_parent = false;  // set by the batch loop while the operation is in ph_parent_lock
   u32 alloc_slab() {
     if ((P.fail_any_every && P.any_allocs.fetch_add(1) % P.fail_any_every == 0) ||
         (P.fail_parent_every && in_parent &&
