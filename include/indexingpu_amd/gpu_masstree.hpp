@@ -963,12 +963,15 @@ struct gpu_masstree {
   // taking them one after another.  Measured on MI355X: 1M-request batches of 4-slice keys
   // over 64M keys, 2159-2166 Mops/s at depth 4 against 2012-2016 at depth 1.  Batches of
   // 16-slice keys under 12-slice shared prefixes (a walk through many trie layers) ran 7%
-  // slower at depth 4, so they, and larger batches, use the tuned depth.
+  // slower at depth 4, so they, and larger batches, use the tuned depth.  With a shortcut
+  // table the chains are short enough that the tuned depth (1) wins there too: 2,480 Mops/s
+  // against 2,412-2,434 at depth 4 on the same 1M-request batches.
   static constexpr uint64_t kSmallFindsPass = 1ull << 21;
   static constexpr uint32_t kShortKey = 4;
   void launch_finds_pass(const indexingpu_amd::batch_args& a, hipStream_t stream,
                          uint64_t work = 0) {
-    const bool small = (work ? work : a.n) <= kSmallFindsPass && a.max_key_length <= kShortKey;
+    const bool small = (work ? work : a.n) <= kSmallFindsPass && a.max_key_length <= kShortKey &&
+                       !shortcut_;
     const uint32_t depth = small ? 4u : find_depth_;
     switch (find_tile_) {
       case 1: launch_narrow<1>(a, stream, true, work, depth); break;
