@@ -757,8 +757,13 @@ struct gpu_masstree {
   }
 
   indexingpu_amd::tree_view view() const {
-    return indexingpu_amd::tree_view{allocator_, reclaimer_, err_, root_index_, shortcut_,
-                                     shortcut_ ? 32u - shortcut_bits_ : 0u};
+    indexingpu_amd::tree_view v{allocator_, reclaimer_, err_, root_index_, shortcut_,
+                                shortcut_ ? 32u - shortcut_bits_ : 0u};
+    if (shortcut_) {
+      v.shortcut_border = shortcut_ + (1ull << shortcut_capacity_bits_);
+      v.shortcut_epoch = shortcut_ + (2ull << shortcut_capacity_bits_);
+    }
+    return v;
   }
 
   // The layer-0 shortcut table of the read-only descents (tree_view::shortcut): built
@@ -790,17 +795,19 @@ struct gpu_masstree {
     uint32_t bits = 18;
     while (bits < kShortcutMaxBits && (1ull << bits) < size) bits++;
     if (bits > shortcut_capacity_bits_) {
+      // level-1 table, border-level table (2^capacity entries each), built-at merge count
       IGPU_HIP_TRY(hipDeviceSynchronize());
       if (shortcut_) IGPU_HIP_TRY(hipFree(shortcut_));
       shortcut_ = nullptr;
-      IGPU_HIP_TRY(hipMalloc(&shortcut_, (4ull << bits)));
+      IGPU_HIP_TRY(hipMalloc(&shortcut_, (8ull << bits) + 256));
       shortcut_capacity_bits_ = bits;
     }
     const uint32_t entries = 1u << bits;
     hipLaunchKernelGGL(indexingpu_amd::shortcut_build_kernel, dim3((entries + 255) / 256),
                        dim3(256), 0, stream, indexingpu_amd::tree_view{allocator_, reclaimer_,
                                                                        err_, root_index_},
-                       shortcut_, entries, 32u - bits);
+                       shortcut_, shortcut_ + (1ull << shortcut_capacity_bits_),
+                       shortcut_ + (2ull << shortcut_capacity_bits_), entries, 32u - bits);
     shortcut_bits_ = bits;
   }
 

@@ -36,7 +36,22 @@ struct tree_view {
   // never rise, so an entry stays a valid B-link starting point after any later write.
   const u32* shortcut;
   u32 shortcut_shift;
+  // Optional border-level twin of the table (find-only descents): entry i is the border
+  // node of layer 0 reached by the smallest first slice of prefix i.  Border nodes can be
+  // merged away and freed, so it is used only while the merge count (err word
+  // kMergeEpochWord) equals the count the table was built at (*shortcut_epoch).
+  const u32* shortcut_border;
+  const u32* shortcut_epoch;
 };
+
+// The table a find-only descent starts from: the border-level one while no merge has
+// happened since it was built, else the level-1 one (or none).
+__device__ __forceinline__ const u32* find_start_table(const tree_view& tv) {
+  if (tv.shortcut_border &&
+      dev::coherent_load(tv.err + kMergeEpochWord) == dev::coherent_load(tv.shortcut_epoch))
+    return tv.shortcut_border;
+  return tv.shortcut;
+}
 
 // Arguments of one batch.  `types` selects per-request kinds (mixed batches); when it
 // is null every request has `kind`.  Find requests write min(len, maxV) value slices
@@ -380,15 +395,18 @@ multi_batch_kernel(tree_view tv, multi_args m) {
 // (the root when the tree has fewer levels) reached from the root by the smallest first
 // slice of prefix i.  Runs with no writer in the launch; the walk moves right past bounds
 // below the slice as any descent does.
-__global__ void shortcut_build_kernel(tree_view tv, u32* table, u32 entries, u32 shift) {
+__global__ void shortcut_build_kernel(tree_view tv, u32* table, u32* border_table,
+                                      u32* built_epoch, u32 entries, u32 shift) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *built_epoch = dev::coherent_load(tv.err + kMergeEpochWord);
   if (i >= entries) return;
   const u32 q = i << shift;
-  u32 node = tv.root;
+  u32 node = tv.root, level1 = tv.root;
   for (u32 guard = 0; guard < 4096; guard++) {
     const u64* p = tv.pool.pool + (u64)node * kSlots;
     const u32 h = slot_lo(p[kMetaSlot]);
-    if (hdr_border(h) || hdr_level(h) <= 1) break;
+    if (hdr_border(h)) break;
+    if (hdr_level(h) <= 1) level1 = node;
     const u32 nk = hdr_nkeys(h);
     u32 next = kInvalid;
     for (u32 j = 0; j < nk && j < kInteriorCap; j++) {
@@ -400,12 +418,13 @@ __global__ void shortcut_build_kernel(tree_view tv, u32* table, u32 entries, u32
     }
     if (next == kInvalid) next = slot_hi(p[kMetaSlot]);  // right of every bound: sibling
     if (next >= tv.pool.num_slabs) {
-      node = tv.root;  // not expected in a quiescent tree: fall back to the root
+      node = level1 = tv.root;  // not expected in a quiescent tree: fall back to the root
       break;
     }
     node = next;
   }
-  table[i] = node;
+  table[i] = level1;
+  border_table[i] = node;
 }
 
 __global__ void zero_kernel(u32* p, u32 n) {

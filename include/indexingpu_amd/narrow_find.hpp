@@ -108,6 +108,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_find_kernel(tree_view tv
   u64 idx[K];
   u32 node[K], layer[K], phase[K], klen[K], q[K], rkind[K];
   u32 kc[K][U];
+  const u32* start_table = find_start_table(tv);
 
   auto set_error = [&](u32 code) {
     if (c.j == 0) atomicCAS(tv.err, 0u, code);
@@ -141,7 +142,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_find_kernel(tree_view tv
       layer[k] = 0;
       phase[k] = kFindDescend;
       q[k] = slice(k, 0);
-      node[k] = tv.shortcut ? tv.shortcut[q[k] >> tv.shortcut_shift] : tv.root;
+      node[k] = start_table ? start_table[q[k] >> tv.shortcut_shift] : tv.root;
       return;
     }
   };
@@ -415,6 +416,9 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   u32 next_kind[K], kind[K];   // (General) request kinds: only finds are answered
   u32 lroot[K], par[K];        // (General) layer root and level-1 node passed (hints)
   bool active[K], q_stale[K];
+  // find-only descents may start at border level; the general ones (write hints want the
+  // level-1 node passed) at level 1
+  const u32* start_table = General ? tv.shortcut : find_start_table(tv);
 
   auto request = [&](u32 k) -> u64 {
     if constexpr (General) return req[k];
@@ -454,7 +458,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       lroot[k] = tv.root;
       par[k] = kInvalid;
     }
-    node[k] = tv.shortcut ? tv.shortcut[q0 >> tv.shortcut_shift] : tv.root;
+    node[k] = start_table ? start_table[q0 >> tv.shortcut_shift] : tv.root;
     layer[k] = 0;
     q_stale[k] = false;
     active[k] = idx[k] < hi;

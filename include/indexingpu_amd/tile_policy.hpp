@@ -102,6 +102,9 @@ struct wave_tile {
     return bcast(won, 0) != 0;
   }
   __device__ __forceinline__ u32 max_op_time() const { return kMaxOpTime; }
+  __device__ __forceinline__ void count_merge() {
+    if (lane == 0) dev::atomic_add(err + kMergeEpochWord, 1u);
+  }
   // the guard's trip record, kept after the error word (layout.hpp)
   __device__ __forceinline__ void store_trip(const u32* w) {
     if (lane == 0)

@@ -1047,6 +1047,7 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       c.unlock_bump(st.node);
       c.unlock_bump(right);
       c.retire(right | kRetiredNode);
+      c.count_merge();  // border-level shortcut tables built before this are retired
       op_finish(st, st.result);
       return true;
     }

@@ -34,6 +34,7 @@ struct shared_pool {
   std::mutex retire_mu;
   std::vector<u32> retired;
   u32 max_op_time = kMaxOpTime;  // the livelock guard's time limit (coarse ticks)
+  std::atomic<u32> merges{0};     // border nodes merged away
   u32 trip[kTripWords] = {};     // the guard's record of the operation it abandoned
   explicit shared_pool(u32 n) : slots((size_t)n * kSlots, 0), num_slabs(n) {}
   u64* slab(u32 idx) { return &slots[(size_t)idx * kSlots]; }
@@ -57,6 +58,7 @@ struct cpu_tile {
     return P.error.compare_exchange_strong(z, code);
   }
   u32 max_op_time() const { return P.max_op_time; }
+  void count_merge() { P.merges.fetch_add(1); }
   void store_trip(const u32* w) {
     for (u32 k = 0; k < kTripWords; k++) P.trip[k] = w[k];
   }

@@ -469,10 +469,12 @@ def test_small_fanout(tile, fanout):
 @pytest.mark.parametrize("find_tile", [2, 8])
 def test_find_shortcut_table(find_tile):
     """Read-only descents of a tree of more than 2^18 keys start at the layer-0 shortcut
-    table (level-1 nodes by first-slice prefix).  Finds of present and absent keys, a split
-    mixed batch (its read-only pass and write hints use the table), then erases that merge
-    nodes and inserts that split them while the table stays as built (stale), and finds
-    again: every result equals the oracle's, and equals the same calls with the table off."""
+    table (level-1 nodes by first-slice prefix; find-only descents at its border-level twin
+    while no node has merged since it was built).  Finds of present and absent keys, a split
+    mixed batch (its read-only pass and write hints use the table), finds after its inserts
+    split border nodes, erases of a contiguous range that merge nodes, then inserts that
+    split them while the table stays as built (stale), and finds again: every result equals
+    the oracle's, and equals the same calls with the table off."""
     rng = np.random.default_rng(23)
     n, mk, mv = 1_300_000, 4, 2
     keys, lens = W.unique_random_keys(rng, n, 1, mk)
@@ -501,6 +503,17 @@ def test_find_shortcut_table(find_tile):
     for a, b in zip(g.mixed(types, keys[idx], lens[idx], v, vl, mk, mv),
                     o.mixed(types, keys[idx], lens[idx], v.copy(), vl, mk, mv)):
         _eq(a, b)
+    # finds through the border-level table after those inserts split its nodes
+    for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        _eq(a, b)
+    # erase every key whose first slice is below 2^28 (a contiguous range: its border
+    # nodes empty and merge, after which finds fall back from the border-level table)
+    low = np.nonzero(keys[:, 0] < (1 << 28))[0]
+    _eq(g.erase(keys[low], lens[low], mk), o.erase(keys[low], lens[low], mk))
+    for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        _eq(a, b)
+    _eq(g.insert(keys[low], lens[low], vals[low], None, mk, mv),
+        o.insert(keys[low], lens[low], vals[low], None, mk, mv))
     # more erases and inserts, then finds through the table as built before them
     _eq(g.erase(keys[era[:50_000]], lens[era[:50_000]], mk), o.erase(keys[era[:50_000]], lens[era[:50_000]], mk))
     _eq(g.insert(keys[era], lens[era], vals[era], None, mk, mv), o.insert(keys[era], lens[era], vals[era], None, mk, mv))
