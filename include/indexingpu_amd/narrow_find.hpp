@@ -392,6 +392,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
                                                                         u32* park) {
   constexpr u32 S = kSlots / T;  // slots per lane
   constexpr u32 kNone = 32;      // no qualifying slot in this lane's piece
+  constexpr bool Ring = !General && K <= T;  // positions from the wave's ring (below)
   const tile_xchg<T> x;
   const u64* pool = tv.pool.pool;
   const u32 nslabs = tv.pool.num_slabs;
@@ -465,23 +466,102 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
     fetch(k, idx[k] + stride);
   };
 
+  // (Ring) The wave takes the batch in chunks of 64 positions: chunk c is positions
+  // 64c .. 64c + 63, and the wave's chunks are its wave number plus multiples of the
+  // grid's wave count.  Lane l holds the first slice and length of position l of the
+  // current chunk and of the next one (loaded one chunk ahead, one coalesced load each),
+  // and a tile whose lookup ends takes the next position of the ring from the lane that
+  // holds it.  Each key row and length line is then read once per chunk, where tiles
+  // that step through their own positions (the other instantiations) read the same
+  // lines again at different times.  A round of takes asks for at most 64 positions
+  // (K <= T), so it stays within the two chunks held.
+  const u32 lane = threadIdx.x & 63u;
+  const u64 waves = (u64)gridDim.x * (kBlockThreads / 64);
+  u64 cur = (u64)blockIdx.x * (kBlockThreads / 64) + threadIdx.x / 64, nxt = cur + waves;
+  u32 ring_q = 0, ring_len = 0, next_ring_q = 0, next_ring_len = 0;
+  u32 head = 0;          // positions of the current chunk already taken
+  bool want[K];          // lookup k has ended and takes a new position
+  auto ring_load = [&](u64 chunk, u32& rq, u32& rl) {
+    const u64 p = chunk * 64 + lane;
+    if (p < hi) {
+      rl = a.key_lengths ? a.key_lengths[p] : maxK;
+      rq = a.keys[p * (u64)maxK];
+    }
+  };
+  auto take = [&]() {
+    u32 tq[K], tl[K];
+    u64 tpos[K];
+    // every position and ring value first, then the loads (a ring value read after a
+    // load was issued would wait for it)
 #pragma unroll
-  for (u32 k = 0; k < K; k++) {
-    idx[k] = lo + first + k;
-    fetch(k, idx[k]);
-    start(k);
+    for (u32 k = 0; k < K; k++) {
+      const u64 b = __ballot(want[k] && x.j == 0);  // one bit per taking tile
+      const u32 slot = head + (u32)__popcll(b & ((1ull << x.base) - 1ull));
+      const int src = (int)((slot & 63u) << 2);
+      const u32 qc = (u32)__builtin_amdgcn_ds_bpermute(src, (int)ring_q);
+      const u32 qn = (u32)__builtin_amdgcn_ds_bpermute(src, (int)next_ring_q);
+      const u32 lc = (u32)__builtin_amdgcn_ds_bpermute(src, (int)ring_len);
+      const u32 ln = (u32)__builtin_amdgcn_ds_bpermute(src, (int)next_ring_len);
+      const bool in_cur = slot < 64;
+      tq[k] = in_cur ? qc : qn;
+      tl[k] = in_cur ? lc : ln;
+      tpos[k] = (in_cur ? cur : nxt) * 64 + (slot & 63u);
+      head += (u32)__popcll(b);
+    }
+    if (head >= 64) {  // the current chunk is used up: the next one becomes current
+      head -= 64;
+      asm volatile("v_mov_b32 %0, %1" : "=v"(ring_q) : "v"(next_ring_q));
+      asm volatile("v_mov_b32 %0, %1" : "=v"(ring_len) : "v"(next_ring_len));
+      cur = nxt;
+      nxt += waves;
+      ring_load(nxt, next_ring_q, next_ring_len);
+    }
+#pragma unroll
+    for (u32 k = 0; k < K; k++) {
+      if (!want[k]) continue;
+      want[k] = false;
+      idx[k] = tpos[k];
+      active[k] = tpos[k] < hi;
+      klen[k] = tl[k];
+      q[k] = tq[k];
+      node[k] = start_table ? start_table[tq[k] >> tv.shortcut_shift] : tv.root;
+      layer[k] = 0;
+      q_stale[k] = false;
+    }
+  };
+
+  if constexpr (Ring) {
+    ring_load(cur, ring_q, ring_len);
+    ring_load(nxt, next_ring_q, next_ring_len);
+#pragma unroll
+    for (u32 k = 0; k < K; k++) want[k] = true;
+    take();
+  } else {
+#pragma unroll
+    for (u32 k = 0; k < K; k++) {
+      idx[k] = lo + first + k;
+      fetch(k, idx[k]);
+      start(k);
+    }
   }
   for (u32 guard = 0; guard < (1u << 24); guard++) {
     bool any = false;
 #pragma unroll
     for (u32 k = 0; k < K; k++) any |= active[k];
-    if (!any) break;
+    if constexpr (Ring) {
+      if (__ballot(any) == 0) break;  // the ring's lanes stay in the loop together
+    } else {
+      if (!any) break;
+    }
     // node[k] is a pool index here (the root after start; a step that leaves an index
     // outside the pool finishes the lookup), so every tile loads unconditionally.  A
     // lookup that entered a new trie layer loads that layer's slice beside its node.
     node_piece<S> n[K];
 #pragma unroll
     for (u32 k = 0; k < K; k++) {
+      if constexpr (Ring) {
+        if (!active[k]) continue;  // an idle tile of the ring loads nothing
+      }
       if (layer[k] != 0 && q_stale[k]) {
         q[k] = a.keys[request(k) * (u64)maxK + layer[k]];
         q_stale[k] = false;
@@ -589,10 +669,16 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
             a.out_value_lengths[idx[k]] = len;
           }
         }
-        idx[k] += stride;
-        start(k);
+        if constexpr (Ring) {
+          active[k] = false;
+          want[k] = true;
+        } else {
+          idx[k] += stride;
+          start(k);
+        }
       }
     }
+    if constexpr (Ring) take();
   }
 }
 
