@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
                                                                         u32* park) {
   constexpr u32 S = kSlots / T;  // slots per lane
   constexpr u32 kNone = 32;      // no qualifying slot in this lane's piece
-  constexpr bool Ring = !General && K <= T;  // positions from the wave's ring (below)
+  constexpr bool Ring = K <= T;  // positions from the wave's ring (below)
   const tile_xchg<T> x;
   const u64* pool = tv.pool.pool;
   const u32 nslabs = tv.pool.num_slabs;
@@ -417,9 +417,17 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   u32 next_kind[K], kind[K];   // (General) request kinds: only finds are answered
   u32 lroot[K], par[K];        // (General) layer root and level-1 node passed (hints)
   bool active[K], q_stale[K];
-  // find-only descents may start at border level; the general ones (write hints want the
-  // level-1 node passed) at level 1
-  const u32* start_table = General ? tv.shortcut : find_start_table(tv);
+  // finds may start at border level (find_start_table); the write requests of the general
+  // passes (their start hints want the level-1 node passed) at level 1
+  const u32* find_table = find_start_table(tv);
+  const u32* write_table = tv.shortcut;
+  // the node a lookup of first slice q0 and request kind kd starts at
+  auto start_node = [&](u32 q0, u32 kd) -> u32 {
+    bool is_find = true;
+    if constexpr (General) is_find = a.types ? kd == op_find : a.kind == op_find;
+    const u32* t = is_find ? find_table : write_table;
+    return t ? t[q0 >> tv.shortcut_shift] : tv.root;
+  };
 
   auto request = [&](u32 k) -> u64 {
     if constexpr (General) return req[k];
@@ -449,47 +457,52 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
     asm volatile("v_mov_b32 %0, %1" : "=v"(q0) : "v"(next_q[k]));
     klen[k] = L;
     q[k] = q0;
+    u32 kd = op_find;
     if constexpr (General) {
       req[k] = next_req[k];
       if (a.types) {
-        u32 kd;
         asm volatile("v_mov_b32 %0, %1" : "=v"(kd) : "v"(next_kind[k]));
         kind[k] = kd;
       }
       lroot[k] = tv.root;
       par[k] = kInvalid;
     }
-    node[k] = start_table ? start_table[q0 >> tv.shortcut_shift] : tv.root;
+    node[k] = start_node(q0, kd);
     layer[k] = 0;
     q_stale[k] = false;
     active[k] = idx[k] < hi;
     fetch(k, idx[k] + stride);
   };
 
-  // (Ring) The wave takes the batch in chunks of 64 positions: chunk c is positions
-  // 64c .. 64c + 63, and the wave's chunks are its wave number plus multiples of the
-  // grid's wave count.  Lane l holds the first slice and length of position l of the
-  // current chunk and of the next one (loaded one chunk ahead, one coalesced load each),
-  // and a tile whose lookup ends takes the next position of the ring from the lane that
-  // holds it.  Each key row and length line is then read once per chunk, where tiles
-  // that step through their own positions (the other instantiations) read the same
-  // lines again at different times.  A round of takes asks for at most 64 positions
-  // (K <= T), so it stays within the two chunks held.
+  // (Ring) The wave takes the launch's positions in chunks of 64: chunk c is positions
+  // lo + 64c .. lo + 64c + 63, and the wave's chunks are its wave number plus multiples of
+  // the grid's wave count.  Lane l holds the first slice, length and (General) kind of
+  // the request at position l of the current chunk and of the next one (loaded one chunk
+  // ahead, one coalesced load each in array order), and a tile whose lookup ends takes
+  // the next position of the ring from the lane that holds it.  Each key row and length
+  // line is then read once per chunk, where tiles that step through their own positions
+  // (the other instantiations) read the same lines again at different times.  A round of
+  // takes asks for at most 64 positions (K <= T), so it stays within the two chunks held.
   const u32 lane = threadIdx.x & 63u;
   const u64 waves = (u64)gridDim.x * (kBlockThreads / 64);
   u64 cur = (u64)blockIdx.x * (kBlockThreads / 64) + threadIdx.x / 64, nxt = cur + waves;
-  u32 ring_q = 0, ring_len = 0, next_ring_q = 0, next_ring_len = 0;
+  u32 ring_q = 0, ring_len = 0, ring_kd = 0, next_ring_q = 0, next_ring_len = 0,
+      next_ring_kd = 0;
   u32 head = 0;          // positions of the current chunk already taken
   bool want[K];          // lookup k has ended and takes a new position
-  auto ring_load = [&](u64 chunk, u32& rq, u32& rl) {
-    const u64 p = chunk * 64 + lane;
+  auto ring_load = [&](u64 chunk, u32& rq, u32& rl, u32& rk) {
+    const u64 p = lo + chunk * 64 + lane;
     if (p < hi) {
-      rl = a.key_lengths ? a.key_lengths[p] : maxK;
-      rq = a.keys[p * (u64)maxK];
+      const u64 c = General ? request_index(a, p) : p;
+      rl = a.key_lengths ? a.key_lengths[c] : maxK;
+      rq = a.keys[c * (u64)maxK];
+      if constexpr (General) {
+        if (a.types) rk = a.types[c];
+      }
     }
   };
   auto take = [&]() {
-    u32 tq[K], tl[K];
+    u32 tq[K], tl[K], tk[K];
     u64 tpos[K];
     // every position and ring value first, then the loads (a ring value read after a
     // load was issued would wait for it)
@@ -505,16 +518,27 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       const bool in_cur = slot < 64;
       tq[k] = in_cur ? qc : qn;
       tl[k] = in_cur ? lc : ln;
-      tpos[k] = (in_cur ? cur : nxt) * 64 + (slot & 63u);
+      tk[k] = op_find;
+      if constexpr (General) {
+        if (a.types) {
+          const u32 kc = (u32)__builtin_amdgcn_ds_bpermute(src, (int)ring_kd);
+          const u32 kn = (u32)__builtin_amdgcn_ds_bpermute(src, (int)next_ring_kd);
+          tk[k] = in_cur ? kc : kn;
+        }
+      }
+      tpos[k] = lo + (in_cur ? cur : nxt) * 64 + (slot & 63u);
       head += (u32)__popcll(b);
     }
     if (head >= 64) {  // the current chunk is used up: the next one becomes current
       head -= 64;
       asm volatile("v_mov_b32 %0, %1" : "=v"(ring_q) : "v"(next_ring_q));
       asm volatile("v_mov_b32 %0, %1" : "=v"(ring_len) : "v"(next_ring_len));
+      if constexpr (General) {
+        if (a.types) asm volatile("v_mov_b32 %0, %1" : "=v"(ring_kd) : "v"(next_ring_kd));
+      }
       cur = nxt;
       nxt += waves;
-      ring_load(nxt, next_ring_q, next_ring_len);
+      ring_load(nxt, next_ring_q, next_ring_len, next_ring_kd);
     }
 #pragma unroll
     for (u32 k = 0; k < K; k++) {
@@ -524,15 +548,21 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
       active[k] = tpos[k] < hi;
       klen[k] = tl[k];
       q[k] = tq[k];
-      node[k] = start_table ? start_table[tq[k] >> tv.shortcut_shift] : tv.root;
+      if constexpr (General) {
+        req[k] = request_index(a, tpos[k]);
+        kind[k] = tk[k];
+        lroot[k] = tv.root;
+        par[k] = kInvalid;
+      }
+      node[k] = start_node(tq[k], tk[k]);
       layer[k] = 0;
       q_stale[k] = false;
     }
   };
 
   if constexpr (Ring) {
-    ring_load(cur, ring_q, ring_len);
-    ring_load(nxt, next_ring_q, next_ring_len);
+    ring_load(cur, ring_q, ring_len, ring_kd);
+    ring_load(nxt, next_ring_q, next_ring_len, next_ring_kd);
 #pragma unroll
     for (u32 k = 0; k < K; k++) want[k] = true;
     take();
