@@ -716,6 +716,14 @@ struct gpu_masstree {
     refresh_shortcut(stream);
     return {shortcut_, shortcut_ ? 32u - shortcut_bits_ : 0u};
   }
+  // Its border-level twin and the merge count it was built at (tree_view::shortcut_border,
+  // shortcut_epoch), null when there is none; same shift.  Only for descents that no
+  // write of the index runs beside (a kernel checks the merge count once, at its start).
+  std::pair<const uint32_t*, const uint32_t*> find_shortcut_border() const {
+    if (!shortcut_) return {nullptr, nullptr};
+    return {shortcut_ + (1ull << shortcut_capacity_bits_),
+            shortcut_ + (2ull << shortcut_capacity_bits_)};
+  }
 
   size_type root_index_ = 0;
   device_allocator_instance_type allocator_;
@@ -759,10 +767,9 @@ struct gpu_masstree {
   indexingpu_amd::tree_view view() const {
     indexingpu_amd::tree_view v{allocator_, reclaimer_, err_, root_index_, shortcut_,
                                 shortcut_ ? 32u - shortcut_bits_ : 0u};
-    if (shortcut_) {
-      v.shortcut_border = shortcut_ + (1ull << shortcut_capacity_bits_);
-      v.shortcut_epoch = shortcut_ + (2ull << shortcut_capacity_bits_);
-    }
+    const auto border = find_shortcut_border();
+    v.shortcut_border = border.first;
+    v.shortcut_epoch = border.second;
     return v;
   }
 

@@ -161,10 +161,12 @@ class masstree_server {
     IGPU_HIP_TRY(hipMalloc(&census_dev_, 8));
     IGPU_HIP_TRY(hipHostMalloc(&census_host_, 8, hipHostMallocDefault));
     // read-only passes of ordered waves start at the tree's layer-0 shortcut table (built
-    // here if due; it stays correct while the waves write)
+    // here if due; it stays correct while the waves write), their finds at its
+    // border-level twin while no node has merged (no write runs beside those passes)
     if (ordered_) {
       IGPU_HIP_TRY(hipDeviceSynchronize());
       shortcut_ = tree_.find_shortcut_table(ops_stream_);
+      shortcut_border_ = tree_.find_shortcut_border();
       IGPU_HIP_TRY(hipStreamSynchronize(ops_stream_));
     }
   }
@@ -370,7 +372,8 @@ class masstree_server {
 
   tree_view view_with(int list) const {
     return tree_view{tree_.allocator_, lists_[list].rec->get_device_instance(), tree_.err_,
-                     tree_.root_index_, shortcut_.first, shortcut_.second};
+                     tree_.root_index_, shortcut_.first, shortcut_.second,
+                     shortcut_border_.first, shortcut_border_.second};
   }
 
   void launch(const serve_arena& a, const serve_request* req, const wave& w, size_t s,
@@ -555,6 +558,8 @@ class masstree_server {
 
   Tree& tree_;
   std::pair<const uint32_t*, uint32_t> shortcut_{nullptr, 0u};  // layer-0 shortcut table
+  // its border-level twin and built-at merge count (ordered waves' read-only passes)
+  std::pair<const uint32_t*, const uint32_t*> shortcut_border_{nullptr, nullptr};
   serve_config cfg_;
   static constexpr uint32_t kOrderedTile = 2;  // lanes per lookup of the read-only pass
   bool ordered_ = false;
