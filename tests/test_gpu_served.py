@@ -14,7 +14,9 @@ from indexingpu_amd.serve import MasstreeServer, latency_summary
 
 pytestmark = pytest.mark.gpu
 
-N_PRELOAD = 200_000
+# above 2^18 keys: the index builds its layer-0 shortcut tables, which the ordered waves'
+# read-only passes start from (their finds at the border-level one)
+N_PRELOAD = 300_000
 
 
 def _loaded(wl):
@@ -63,6 +65,8 @@ def test_same_result_alone_and_in_the_mix(streams, wave_ops, epoch, ordered):
     ga, gb = _loaded(wl), _loaded(wl)
     sv = MasstreeServer(ga.t, streams=streams, max_wave_ops=wave_ops, max_wave_scans=256,
                         ring_rows=n_scans * mc, epoch_waves=epoch, ordered=ordered)
+    if ordered:
+        assert ga.t.find_shortcut_bits() > 0  # the ordered waves start from the tables
     out_mix, st = sv.run(mix, req)
     ga.t.check_device_error()
     assert st["device_error"] == 0
