@@ -953,7 +953,9 @@ struct gpu_masstree {
   template <uint32_t T>
   void launch_descend_only(const indexingpu_amd::batch_args& a, hipStream_t stream) {
     constexpr uint32_t per_block = indexingpu_amd::kBlockThreads / T;
-    auto kernel = indexingpu_amd::narrow_descend_kernel<T, 1, true>;
+    // a scattered order (growth stages) reads random key rows either way: no ring
+    auto kernel = a.perm_mul ? indexingpu_amd::narrow_descend_kernel<T, 1, true, false>
+                             : indexingpu_amd::narrow_descend_kernel<T, 1, true>;
     const uint64_t want = (a.n + per_block - 1) / per_block;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, indexingpu_amd::resident_blocks(kernel));
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(indexingpu_amd::kBlockThreads), 0, stream,
@@ -995,6 +997,12 @@ struct gpu_masstree {
       default: launch_narrow<2>(a, stream, true, work, depth); break;
     }
   }
+  // Smallest batch whose general read-only pass takes positions from the wave's ring.  A
+  // wave of 2-lane tiles holds one lookup per tile but takes 64-position chunks, so a batch
+  // the resident grid covers in one round runs in two; on the 16-slice batched workload
+  // (batches of 1K..1M requests) the ring on every batch measured 4% slower, while the 1M
+  // mixed batches gained.
+  static constexpr uint64_t kGeneralRingMin = 1ull << 19;
   template <uint32_t T>
   void launch_narrow(const indexingpu_amd::batch_args& a, hipStream_t stream,
                      bool two_pass = false, uint64_t work = 0, uint32_t depth = 0,
@@ -1017,7 +1025,15 @@ struct gpu_masstree {
       // plain find batches take the find-only kernels; the finds pass of a mixed batch
       // (kinds, ranges, write hints) the general ones
       const bool general = a.types || a.range || a.hints || a.perm_mul;
-      if (general) {
+      if (general && (a.range || a.perm_mul || a.n < kGeneralRingMin)) {
+        // the batched form's device-read ranges, scattered orders and batches below
+        // kGeneralRingMin requests: tiles step through their own positions
+        switch (depth) {
+          case 1: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 1, true, false>, tiles); break;
+          case 4: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 4, true, false>, tiles * 4); break;
+          default: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 2, true, false>, tiles * 2); break;
+        }
+      } else if (general) {
         switch (depth) {
           case 1: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 1, true>, tiles); break;
           case 4: launch_descend(indexingpu_amd::narrow_descend_kernel<T, 4, true>, tiles * 4); break;

@@ -387,12 +387,15 @@ struct tile_xchg {
 // (only finds are answered), the device-read range of the batched form, a scattered
 // request order (request_index), and the start hints of write requests.  The two are
 // separate instantiations, so find batches carry none of it.
-template <u32 T, u32 K, bool General = false>
+template <u32 T, u32 K, bool General = false, bool UseRing = true>
 __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view tv, batch_args a,
                                                                         u32* park) {
   constexpr u32 S = kSlots / T;  // slots per lane
   constexpr u32 kNone = 32;      // no qualifying slot in this lane's piece
-  constexpr bool Ring = K <= T;  // positions from the wave's ring (below)
+  // positions from the wave's ring (below) unless the launch asks for tiles stepping
+  // through their own positions (UseRing = false: the batched form's device-read ranges and
+  // scattered orders, whose key rows the ring would not read any fewer times)
+  constexpr bool Ring = UseRing && K <= T;
   const tile_xchg<T> x;
   const u64* pool = tv.pool.pool;
   const u32 nslabs = tv.pool.num_slabs;
