@@ -393,8 +393,9 @@ __global__ void __launch_bounds__(kBlockThreads) narrow_descend_kernel(tree_view
   constexpr u32 S = kSlots / T;  // slots per lane
   constexpr u32 kNone = 32;      // no qualifying slot in this lane's piece
   // positions from the wave's ring (below) unless the launch asks for tiles stepping
-  // through their own positions (UseRing = false: the batched form's device-read ranges and
-  // scattered orders, whose key rows the ring would not read any fewer times)
+  // through their own positions (UseRing = false: the host's choice for the batched form's
+  // device-read ranges, scattered orders and general batches too small to fill the grid
+  // twice, where the ring reads no fewer key lines or runs in more rounds)
   constexpr bool Ring = UseRing && K <= T;
   const tile_xchg<T> x;
   const u64* pool = tv.pool.pool;
