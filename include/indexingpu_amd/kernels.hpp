@@ -36,10 +36,10 @@ struct tree_view {
   // never rise, so an entry stays a valid B-link starting point after any later write.
   const u32* shortcut;
   u32 shortcut_shift;
-  // Optional border-level twin of the table (find-only descents): entry i is the border
-  // node of layer 0 reached by the smallest first slice of prefix i.  Border nodes can be
-  // merged away and freed, so it is used only while the merge count (err word
-  // kMergeEpochWord) equals the count the table was built at (*shortcut_epoch).
+  // Optional border-level twin of the table (finds): entry i is the border node of layer 0
+  // reached by the smallest first slice of prefix i.  Border nodes can be merged away and
+  // freed, so it is used only while the count of layer-0 merges (err word kMergeEpochWord)
+  // equals the count the table was built at (*shortcut_epoch).
   const u32* shortcut_border;
   const u32* shortcut_epoch;
 };

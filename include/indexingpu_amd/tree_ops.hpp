@@ -1047,7 +1047,9 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       c.unlock_bump(st.node);
       c.unlock_bump(right);
       c.retire(right | kRetiredNode);
-      c.count_merge();  // border-level shortcut tables built before this are retired
+      // a border-level shortcut table (layer-0 border nodes) built before this merge of a
+      // layer-0 node is retired; merges in deeper layers leave it valid
+      if (st.layer == 0) c.count_merge();
       op_finish(st, st.result);
       return true;
     }

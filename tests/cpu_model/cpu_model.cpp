@@ -202,7 +202,7 @@ uint64_t cm_unlinked(void* h) {
   return w.run(M.root).unlinked;
 }
 // Reachable nodes (which = 0), border nodes (1), empty non-root border nodes (2), the
-// largest border entry count (3) and interior child count (4); merges counted so far (5).
+// largest border entry count (3) and interior child count (4); merges of layer-0 border nodes so far (5).
 uint64_t cm_nodes(void* h, uint32_t which) {
   model& M = *static_cast<model*>(h);
   if (which == 5) return M.pool.merges.load();

@@ -125,7 +125,7 @@ class CpuModel:
 
     def nodes(self, which=0):
         """Reachable nodes (0), border nodes (1), empty non-root border nodes (2), the
-        largest border entry count (3) or interior child count (4); merges counted (5)."""
+        largest border entry count (3) or interior child count (4); merges of layer-0 border nodes counted (5)."""
         return int(self.lib.cm_nodes(self.h, which))
 
     @property

@@ -312,6 +312,31 @@ def test_hinted_writes_on_few_hot_nodes(nthreads):
         assert b.items() == o.items()
 
 
+@pytest.mark.parametrize("nthreads", [0, 8])
+def test_merge_count_is_layer_zero_only(nthreads):
+    """The merge count a border-level shortcut table is checked against (layer-0 border nodes
+    only: the table holds no other node) stays put when erases empty and merge border nodes
+    of a deeper layer: every key shares its first slice, so every border node that merges
+    lies in layer 1.  Results equal the oracle's and the node count falls."""
+    rng = np.random.default_rng(73)
+    n, mk, mv = 4000, 2, 1
+    keys = np.zeros((n, mk), np.uint32)
+    keys[:, 0] = 0x12345678
+    keys[:, 1] = rng.permutation(1 << 20)[:n].astype(np.uint32)
+    lens = np.full(n, 2, np.uint32)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    b = S.CpuModelBackend(nthreads=nthreads, ntiles=64)
+    o = S.OracleBackend()
+    _eq = np.array_equal
+    assert _eq(b.insert(keys, lens, vals, None, mk, mv), o.insert(keys, lens, vals, None, mk, mv))
+    full_border = b.m.nodes(1)
+    e = np.nonzero(keys[:, 1] < (1 << 19))[0]  # a contiguous half of layer 1
+    assert _eq(b.erase(keys[e], lens[e], mk), o.erase(keys[e], lens[e], mk))
+    assert b.items() == o.items()
+    assert b.m.nodes(1) < full_border * 3 // 4  # layer-1 border nodes merged away
+    assert b.m.nodes(5) == 0
+
+
 @pytest.mark.parametrize("hinted", [False, True], ids=["from_root", "start_hints"])
 @pytest.mark.parametrize("merge", [True, False], ids=["merge", "no_merge"])
 @pytest.mark.parametrize("nthreads", [0, 8])
@@ -348,8 +373,8 @@ def test_erase_merges_empty_border_nodes(nthreads, hinted, merge):
         assert b.m.nodes(0) < full // 2 and b.m.nodes(1) < full_border // 2
         # an emptied node stays only as its parent's one child (or when its neighbour was busy)
         assert b.m.nodes(2) <= b.m.nodes(0) - b.m.nodes(1)
-        # merges advanced the count a border-level shortcut table is checked against (the
-        # rest of the fall in border nodes is layers removed whole)
+        # layer-0 merges advanced the count a border-level shortcut table is checked against
+        # (the rest of the fall in border nodes is layers removed whole and deeper merges)
         assert b.m.nodes(5) > 0
     else:
         assert b.m.nodes(0) == full and b.m.nodes(2) > 0 and b.m.nodes(5) == 0
