@@ -218,8 +218,9 @@ int gmt_reserve_find_scratch(gmt_index* index, uint64_t max_requests);
 /* Read-only descents (find, the read-only pass of mixed batches, the start hints of write
  * batches) of a tree of at least 2^18 inserted keys whose root is at level 3 or above
  * start at a layer-0 shortcut table (about one 4-byte entry per key, at most 2^26: the
- * level-1 node holding the smallest key of each prefix of the first slice; find batches
- * start one level lower, at the border node, while no node has merged since the build)
+ * level-1 node holding the smallest key of each prefix of the first slice; finds start one
+ * level lower, at the border node, through a twin table that merges rewrite, or that a
+ * merge it cannot follow retires until the next rebuild)
  * instead of the root: on (the default) or off.  An extension; results do not depend on it.  The table
  * is rebuilt when the insert count has doubled; one left from before later writes is
  * still correct. */

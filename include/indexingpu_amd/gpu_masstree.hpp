@@ -718,8 +718,10 @@ struct gpu_masstree {
   }
   // Its border-level twin and the merge count it was built at (tree_view::shortcut_border,
   // shortcut_epoch), null when there is none; same shift.  Only for descents that no
-  // write of the index runs beside (a kernel checks the merge count once, at its start).
-  std::pair<const uint32_t*, const uint32_t*> find_shortcut_border() const {
+  // write of the index runs beside (a kernel checks the merge count once, at its start);
+  // write launches given it rewrite its entries when they merge nodes away
+  // (shortcut_patch).
+  std::pair<uint32_t*, const uint32_t*> find_shortcut_border() const {
     if (!shortcut_) return {nullptr, nullptr};
     return {shortcut_ + (1ull << shortcut_capacity_bits_),
             shortcut_ + (2ull << shortcut_capacity_bits_)};

@@ -40,7 +40,7 @@ struct tree_view {
   // reached by the smallest first slice of prefix i.  Border nodes can be merged away and
   // freed, so it is used only while the count of layer-0 merges (err word kMergeEpochWord)
   // equals the count the table was built at (*shortcut_epoch).
-  const u32* shortcut_border;
+  u32* shortcut_border;  // (merges of the launch's writes keep it current: shortcut_patch)
   const u32* shortcut_epoch;
 };
 
@@ -129,6 +129,8 @@ template <u32 T, bool Validate>
 __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5)))
 batch_kernel(tree_view tv, batch_args a) {
   wave_tile<T, Validate> c(tv.pool, tv.retire, tv.err, tv.root);
+  c.sc_border = tv.shortcut_border;
+  c.sc_shift = tv.shortcut_shift;
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
   u64 i = (u64)blockIdx.x * tiles_per_block + threadIdx.x / T;
@@ -264,6 +266,8 @@ template <u32 T>
 __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5)))
 multi_batch_kernel(tree_view tv, multi_args m) {
   wave_tile<T, true> c(tv.pool, tv.retire, tv.err, tv.root);
+  c.sc_border = tv.shortcut_border;
+  c.sc_shift = tv.shortcut_shift;
   const batch_args& a = m.a;
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;

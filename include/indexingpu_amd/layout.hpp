@@ -155,5 +155,8 @@ constexpr u32 kTripWords = 16;
 // border-level shortcut table is used only while this equals its value when the table was
 // built.
 constexpr u32 kMergeEpochWord = 8;
+// Entries of a border-level shortcut table a merge rewrites at most (the entries naming the
+// node that goes are one run); a longer run retires the table instead.
+constexpr u32 kShortcutPatchMax = 64;
 
 }  // namespace indexingpu_amd

@@ -559,7 +559,7 @@ class masstree_server {
   Tree& tree_;
   std::pair<const uint32_t*, uint32_t> shortcut_{nullptr, 0u};  // layer-0 shortcut table
   // its border-level twin and built-at merge count (ordered waves' read-only passes)
-  std::pair<const uint32_t*, const uint32_t*> shortcut_border_{nullptr, nullptr};
+  std::pair<uint32_t*, const uint32_t*> shortcut_border_{nullptr, nullptr};
   serve_config cfg_;
   static constexpr uint32_t kOrderedTile = 2;  // lanes per lookup of the read-only pass
   bool ordered_ = false;

@@ -105,6 +105,30 @@ struct wave_tile {
   __device__ __forceinline__ void count_merge() {
     if (lane == 0) dev::atomic_add(err + kMergeEpochWord, 1u);
   }
+  // Border-level shortcut table of the launch (tree_view::shortcut_border; null: none) and
+  // its shift.  A merge that removes layer-0 border node `right` into `left` rewrites the
+  // entries naming `right`: they are one run starting at the first prefix above right's
+  // exclusive low bound `low` (an entry names the node holding its prefix's smallest slice
+  // when the table was built, or one left of it; low bounds of surviving nodes never
+  // move).  False when there is no table or the run is longer than kShortcutPatchMax.
+  u32* sc_border = nullptr;
+  u32 sc_shift = 0;
+  __device__ __forceinline__ bool shortcut_patch(u32 right, u32 left, u32 low) {
+    if (!sc_border) return false;
+    u32 ok = 0;
+    if (lane == 0) {
+      const u32 entries = 1u << (32u - sc_shift);
+      u32 i = (low >> sc_shift) + 1u;  // entries at or below low name `left` or nodes left of it
+      u32 n = 0;
+      while (i < entries && n < kShortcutPatchMax && dev::coherent_load(sc_border + i) == right) {
+        dev::coherent_store(sc_border + i, left);
+        i++;
+        n++;
+      }
+      ok = (i >= entries || dev::coherent_load(sc_border + i) != right) ? 1u : 0u;
+    }
+    return bcast(ok, 0) != 0;
+  }
   // the guard's trip record, kept after the error word (layout.hpp)
   __device__ __forceinline__ void store_trip(const u32* w) {
     if (lane == 0)

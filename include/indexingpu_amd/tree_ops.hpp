@@ -1022,6 +1022,8 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       node_t en;
       c.read_node_locked(st.a0, en);
       const u32 eh = c.header(en);
+      // the exclusive low bound of the node that goes: the left one's high key
+      const u32 low = j ? c.high(on) : c.high(en);
       u32 left = other, right = st.a0;
       if (j) {
         c.set_header(on, oold | kLockBit);
@@ -1042,14 +1044,17 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       }
       c.interior_remove(n, j ? j : 1);
       c.write_node(st.node, n);
+      // a border-level shortcut table (its entries are layer-0 border nodes) is rewritten
+      // to name the left node where it named the right one, before the locks go (a merge
+      // that removes the left node next waits for them); when it cannot be (no table
+      // given, or too long a run), the merge is counted and the table retired
+      const bool patched = st.layer == 0 && c.shortcut_patch(right, left, low);
       c.fence_release();
       c.unlock_bump(left);
       c.unlock_bump(st.node);
       c.unlock_bump(right);
       c.retire(right | kRetiredNode);
-      // a border-level shortcut table (layer-0 border nodes) built before this merge of a
-      // layer-0 node is retired; merges in deeper layers leave it valid
-      if (st.layer == 0) c.count_merge();
+      if (st.layer == 0 && !patched) c.count_merge();
       op_finish(st, st.result);
       return true;
     }

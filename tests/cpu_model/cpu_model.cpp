@@ -26,6 +26,7 @@ struct model {
   u32 hinted = 0;  // writes start at hints from a read-only walk (as the GPU write passes)
   u32 no_suffix = 0;  // inserts build layers instead of suffix records (enable_suffix=false)
   u32 erase_flags = 0;  // kReqNoMerge | kReqKeepEmptyLayers | fanout bits for every request
+  std::vector<u32> shortcut;  // border-level shortcut table (cm_build_shortcut)
   explicit model(u32 n) : pool(n) {
     cpu_tile t(pool);
     root = t.alloc_slab();
@@ -217,6 +218,61 @@ uint64_t cm_nodes(void* h, uint32_t which) {
     case 3: return s.max_border_keys;
     default: return s.max_interior_children;
   }
+}
+
+// The layer-0 border node reached from the root by first slice q (the walk of the
+// device's shortcut_build_kernel: the first bound >= q, else the sibling).
+static u32 border_of(model& M, u32 q) {
+  u32 node = M.root;
+  for (u32 guard = 0; guard < 4096; guard++) {
+    const u64* p = M.pool.slab(node);
+    const u32 h = slot_lo(p[kMetaSlot]);
+    if (hdr_border(h)) break;
+    u32 next = kInvalid;
+    for (u32 j = 0; j < hdr_nkeys(h) && j < kInteriorCap; j++) {
+      if (q <= slot_lo(p[j])) {
+        next = child_of(p[j]);
+        break;
+      }
+    }
+    if (next == kInvalid) next = slot_hi(p[kMetaSlot]);
+    if (next >= M.pool.num_slabs) return kInvalid;
+    node = next;
+  }
+  return node;
+}
+// Builds a border-level shortcut table of 2^bits entries (entry i: the border node of
+// first slice i << (32 - bits)); merges of later batches rewrite it (shortcut_patch).
+void cm_build_shortcut(void* h, uint32_t bits) {
+  model& M = *static_cast<model*>(h);
+  const u32 entries = 1u << bits, shift = 32u - bits;
+  M.shortcut.assign(entries, kInvalid);
+  for (u32 i = 0; i < entries; i++) M.shortcut[i] = border_of(M, i << shift);
+  M.pool.sc_border = M.shortcut.data();
+  M.pool.sc_shift = shift;
+}
+// Entries of the table that are no valid start for their prefix: not a live layer-0
+// border node (on the chain of sibling links from the leftmost one), or right of the
+// node that holds the prefix's smallest slice.
+uint64_t cm_check_shortcut(void* h) {
+  model& M = *static_cast<model*>(h);
+  std::vector<u32> chain, highs;
+  std::vector<int64_t> pos(M.pool.num_slabs, -1);
+  for (u32 n = border_of(M, 0); n < M.pool.num_slabs && chain.size() < M.pool.num_slabs;
+       n = slot_hi(M.pool.slab(n)[kMetaSlot])) {
+    pos[n] = (int64_t)chain.size();
+    chain.push_back(n);
+    highs.push_back(slot_lo(M.pool.slab(n)[kHighSlot]));
+  }
+  uint64_t bad = 0;
+  const u32 shift = M.pool.sc_shift;
+  size_t t = 0;  // the chain node holding q: the first whose high is >= q (q ascends)
+  for (size_t i = 0; i < M.shortcut.size(); i++) {
+    const u32 q = (u32)i << shift, x = M.shortcut[i];
+    while (t + 1 < chain.size() && highs[t] < q) t++;
+    if (x >= M.pool.num_slabs || pos[x] < 0 || (size_t)pos[x] > t) bad++;
+  }
+  return bad;
 }
 
 // Runs one batch.  nthreads == 0: interleaved simulation with ntiles tiles and the

@@ -34,7 +34,9 @@ struct shared_pool {
   std::mutex retire_mu;
   std::vector<u32> retired;
   u32 max_op_time = kMaxOpTime;  // the livelock guard's time limit (coarse ticks)
-  std::atomic<u32> merges{0};     // border nodes merged away
+  std::atomic<u32> merges{0};     // layer-0 border nodes merged away without a table rewrite
+  u32* sc_border = nullptr;       // border-level shortcut table (null: none) and its shift
+  u32 sc_shift = 0;
   u32 trip[kTripWords] = {};     // the guard's record of the operation it abandoned
   explicit shared_pool(u32 n) : slots((size_t)n * kSlots, 0), num_slabs(n) {}
   u64* slab(u32 idx) { return &slots[(size_t)idx * kSlots]; }
@@ -59,6 +61,20 @@ struct cpu_tile {
   }
   u32 max_op_time() const { return P.max_op_time; }
   void count_merge() { P.merges.fetch_add(1); }
+  // the device policy's shortcut_patch on the model's table
+  bool shortcut_patch(u32 right, u32 left, u32 low) {
+    if (!P.sc_border) return false;
+    const u32 entries = 1u << (32u - P.sc_shift);
+    u32 i = (low >> P.sc_shift) + 1u;
+    u32 n = 0;
+    while (i < entries && n < kShortcutPatchMax &&
+           __atomic_load_n(P.sc_border + i, __ATOMIC_RELAXED) == right) {
+      __atomic_store_n(P.sc_border + i, left, __ATOMIC_RELAXED);
+      i++;
+      n++;
+    }
+    return i >= entries || __atomic_load_n(P.sc_border + i, __ATOMIC_RELAXED) != right;
+  }
   void store_trip(const u32* w) {
     for (u32 k = 0; k < kTripWords; k++) P.trip[k] = w[k];
   }

@@ -37,6 +37,9 @@ def _load():
         lib.cm_root.restype = u32
         lib.cm_set_lock.argtypes = [vp, u32, u32]
         lib.cm_set_lock.restype = u32
+        lib.cm_build_shortcut.argtypes = [vp, u32]
+        lib.cm_check_shortcut.argtypes = [vp]
+        lib.cm_check_shortcut.restype = u64
         lib.cm_unlinked.argtypes = [vp]
         lib.cm_unlinked.restype = u64
         lib.cm_nodes.argtypes = [vp, u32]
@@ -127,6 +130,14 @@ class CpuModel:
         """Reachable nodes (0), border nodes (1), empty non-root border nodes (2), the
         largest border entry count (3) or interior child count (4); merges of layer-0 border nodes counted (5)."""
         return int(self.lib.cm_nodes(self.h, which))
+
+    def build_shortcut(self, bits):
+        """Builds a border-level shortcut table of 2**bits entries that later merges rewrite."""
+        self.lib.cm_build_shortcut(self.h, bits)
+
+    def check_shortcut(self):
+        """Entries of the table that are no valid start for their prefix (0 when it is sound)."""
+        return int(self.lib.cm_check_shortcut(self.h))
 
     @property
     def unlinked(self):

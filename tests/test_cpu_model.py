@@ -313,6 +313,64 @@ def test_hinted_writes_on_few_hot_nodes(nthreads):
 
 
 @pytest.mark.parametrize("nthreads", [0, 8])
+def test_merges_rewrite_shortcut_table(nthreads):
+    """A border-level shortcut table built on the tree stays a valid start for every prefix
+    while erases empty and merge layer-0 border nodes and inserts split them, batch after
+    batch: each merge rewrites the entries that named the node it removed (the count that
+    would retire the table stays 0), and results equal the oracle's."""
+    rng = np.random.default_rng(75)
+    n, mk, mv = 6000, 2, 1
+    keys, lens = W.unique_random_keys(rng, n, 1, mk)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    b = S.CpuModelBackend(nthreads=nthreads, ntiles=64)
+    o = S.OracleBackend()
+    _eq = np.array_equal
+    assert _eq(b.insert(keys, lens, vals, None, mk, mv), o.insert(keys, lens, vals, None, mk, mv))
+    full_border = b.m.nodes(1)
+    b.m.build_shortcut(10)
+    assert b.m.check_shortcut() == 0
+    gone = ((keys[:, 0] >= (1 << 30)) & (keys[:, 0] < (1 << 31))) | (rng.random(n) < 0.3)
+    e = np.nonzero(gone)[0]
+    assert _eq(b.erase(keys[e], lens[e], mk), o.erase(keys[e], lens[e], mk))
+    assert b.items() == o.items()
+    assert b.m.nodes(1) < full_border - full_border // 8  # layer-0 border nodes merged away
+    assert b.m.nodes(5) == 0
+    assert b.m.check_shortcut() == 0
+    for rnd in range(3):
+        back = rng.permutation(e)[: len(e) // 3]
+        rest = np.setdiff1d(np.arange(n), e)
+        out = rng.permutation(rest)[: len(rest) // 3]
+        assert _eq(b.insert(keys[back], lens[back], vals[back], None, mk, mv),
+                   o.insert(keys[back], lens[back], vals[back], None, mk, mv)), rnd
+        assert _eq(b.erase(keys[out], lens[out], mk), o.erase(keys[out], lens[out], mk)), rnd
+        e = np.union1d(np.setdiff1d(e, back), out)
+        assert b.items() == o.items(), rnd
+        assert b.m.nodes(5) == 0, rnd
+        assert b.m.check_shortcut() == 0, rnd
+
+
+def test_long_table_run_retires_shortcut_table():
+    """When the entries naming the node a merge removes are a run longer than the merge
+    rewrites (here the tree's last border node names every prefix above its keys), the
+    merge is counted instead, which retires the table."""
+    rng = np.random.default_rng(77)
+    n, mk, mv = 600, 1, 1
+    keys = np.sort(rng.choice(1 << 22, size=n, replace=False)).astype(np.uint32).reshape(n, 1)
+    lens = np.ones(n, np.uint32)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    b = S.CpuModelBackend(nthreads=0, ntiles=8)
+    o = S.OracleBackend()
+    _eq = np.array_equal
+    assert _eq(b.insert(keys, lens, vals, None, mk, mv), o.insert(keys, lens, vals, None, mk, mv))
+    b.m.build_shortcut(12)
+    assert b.m.check_shortcut() == 0
+    top = np.arange(n - 40, n)  # the largest keys: the last border nodes empty and merge
+    assert _eq(b.erase(keys[top], lens[top], mk), o.erase(keys[top], lens[top], mk))
+    assert b.items() == o.items()
+    assert b.m.nodes(5) > 0
+
+
+@pytest.mark.parametrize("nthreads", [0, 8])
 def test_merge_count_is_layer_zero_only(nthreads):
     """The merge count a border-level shortcut table is checked against (layer-0 border nodes
     only: the table holds no other node) stays put when erases empty and merge border nodes

@@ -472,9 +472,10 @@ def test_find_shortcut_table(find_tile):
     table (level-1 nodes by first-slice prefix; find-only descents at its border-level twin
     while no node has merged since it was built).  Finds of present and absent keys, a split
     mixed batch (its read-only pass and write hints use the table), finds after its inserts
-    split border nodes, erases of a contiguous range that merge nodes, then inserts that
-    split them while the table stays as built (stale), and finds again: every result equals
-    the oracle's, and equals the same calls with the table off."""
+    split border nodes, erases of short ranges whose merges rewrite the table, erases of a
+    long range that retire it, then inserts that split nodes while the table stays as built
+    (stale), and finds again: every result equals the oracle's, and equals the same calls
+    with the table off."""
     rng = np.random.default_rng(23)
     n, mk, mv = 1_300_000, 4, 2
     keys, lens = W.unique_random_keys(rng, n, 1, mk)
@@ -506,8 +507,18 @@ def test_find_shortcut_table(find_tile):
     # finds through the border-level table after those inserts split its nodes
     for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
         _eq(a, b)
+    # erase the keys of 256 short first-slice ranges (2^16 wide): their border nodes empty
+    # and merge one or two at a time, each merge rewriting the few table entries that named
+    # the node it removed, so finds keep starting at the border-level table
+    small = np.nonzero((keys[:, 0] & ((1 << 24) - 1)) < (1 << 16))[0]
+    _eq(g.erase(keys[small], lens[small], mk), o.erase(keys[small], lens[small], mk))
+    for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        _eq(a, b)
+    _eq(g.insert(keys[small], lens[small], vals[small], None, mk, mv),
+        o.insert(keys[small], lens[small], vals[small], None, mk, mv))
     # erase every key whose first slice is below 2^28 (a contiguous range: its border
-    # nodes empty and merge, after which finds fall back from the border-level table)
+    # nodes empty and merge into long runs of table entries, which retires the
+    # border-level table; finds fall back to the level-1 one)
     low = np.nonzero(keys[:, 0] < (1 << 28))[0]
     _eq(g.erase(keys[low], lens[low], mk), o.erase(keys[low], lens[low], mk))
     for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
