@@ -38,8 +38,9 @@ struct tree_view {
   u32 shortcut_shift;
   // Optional border-level twin of the table (finds): entry i is the border node of layer 0
   // reached by the smallest first slice of prefix i.  Border nodes can be merged away and
-  // freed, so it is used only while the count of layer-0 merges (err word kMergeEpochWord)
-  // equals the count the table was built at (*shortcut_epoch).
+  // freed: a merge rewrites the entries naming the node it removes (shortcut_patch), and
+  // one that cannot is counted (err word kMergeEpochWord); the table is used only while
+  // that count equals the count it was built at (*shortcut_epoch).
   u32* shortcut_border;  // (merges of the launch's writes keep it current: shortcut_patch)
   const u32* shortcut_epoch;
 };

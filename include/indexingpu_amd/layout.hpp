@@ -151,9 +151,9 @@ enum error_code : u32 {
 // key length, the key slice of the current layer, lvl | fresh << 16.
 constexpr u32 kTripOffset = 16;
 constexpr u32 kTripWords = 16;
-// Word of the error area counting merges of layer-0 border nodes (never cleared): a
-// border-level shortcut table is used only while this equals its value when the table was
-// built.
+// Word of the error area counting merges of layer-0 border nodes that did not rewrite the
+// border-level shortcut table (never cleared): the table is used only while this equals its
+// value when the table was built.
 constexpr u32 kMergeEpochWord = 8;
 // Entries of a border-level shortcut table a merge rewrites at most (the entries naming the
 // node that goes are one run); a longer run retires the table instead.
