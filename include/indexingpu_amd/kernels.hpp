@@ -45,8 +45,8 @@ struct tree_view {
   const u32* shortcut_epoch;
 };
 
-// The table a find-only descent starts from: the border-level one while no merge has
-// happened since it was built, else the level-1 one (or none).
+// The table a find starts from: the border-level one while every merge since it was built
+// has rewritten it, else the level-1 one (or none).
 __device__ __forceinline__ const u32* find_start_table(const tree_view& tv) {
   if (tv.shortcut_border &&
       dev::coherent_load(tv.err + kMergeEpochWord) == dev::coherent_load(tv.shortcut_epoch))

@@ -162,7 +162,8 @@ class masstree_server {
     IGPU_HIP_TRY(hipHostMalloc(&census_host_, 8, hipHostMallocDefault));
     // read-only passes of ordered waves start at the tree's layer-0 shortcut table (built
     // here if due; it stays correct while the waves write), their finds at its
-    // border-level twin while no node has merged (no write runs beside those passes)
+    // border-level twin, which their write passes rewrite when they merge nodes (no write
+    // runs beside those read-only passes)
     if (ordered_) {
       IGPU_HIP_TRY(hipDeviceSynchronize());
       shortcut_ = tree_.find_shortcut_table(ops_stream_);
