@@ -106,13 +106,20 @@ struct wave_tile {
     if (lane == 0) dev::atomic_add(err + kMergeEpochWord, 1u);
   }
   // Border-level shortcut table of the launch (tree_view::shortcut_border; null: none) and
-  // its shift.  A merge that removes layer-0 border node `right` into `left` rewrites the
-  // entries naming `right`: they are one run starting at the first prefix above right's
-  // exclusive low bound `low` (an entry names the node holding its prefix's smallest slice
-  // when the table was built, or one left of it; low bounds of surviving nodes never
-  // move).  False when there is no table or the run is longer than kShortcutPatchMax.
+  // its shift.  A merge that removes layer-0 border node `right` (exclusive low bound
+  // `low`) rewrites the run of entries naming it, which starts at the first prefix above
+  // `low`, to `target` (tree_ops.hpp, ph_merge_lock).  shortcut_prev: the entry of the
+  // prefix holding `low` (kInvalid: no table).  shortcut_patch: false when there is no
+  // table or the run is longer than kShortcutPatchMax.
   u32* sc_border = nullptr;
   u32 sc_shift = 0;
+  __device__ __forceinline__ u32 slab_count() const { return pool.num_slabs; }
+  __device__ __forceinline__ u32 shortcut_prev(u32 low) const {
+    if (!sc_border) return kInvalid;
+    u32 v = 0;
+    if (lane == 0) v = dev::coherent_load(sc_border + (low >> sc_shift));
+    return bcast(v, 0);
+  }
   __device__ __forceinline__ bool shortcut_patch(u32 right, u32 left, u32 low) {
     if (!sc_border) return false;
     u32 ok = 0;

@@ -1044,12 +1044,30 @@ IGPU_HD bool op_step(C& c, const request& rq, S& st) {
       }
       c.interior_remove(n, j ? j : 1);
       c.write_node(st.node, n);
-      // a border-level shortcut table (its entries are layer-0 border nodes) is rewritten
-      // to name the left node where it named the right one, before the locks go (a merge
-      // that removes the left node next waits for them); when it cannot be (no table
-      // given, or too long a run), the merge is counted and the table retired
-      const bool patched = st.layer == 0 && c.shortcut_patch(right, left, low);
+      // A border-level shortcut table (entries are layer-0 border nodes; every node some
+      // entry names is named by one run of entries that starts at the first prefix above
+      // the node's low bound) is rewritten where it named the right node: that run starts
+      // right after the entry of the prefix holding `low`, and takes that entry's node
+      // (`prev`: the left node or one left of it, a valid start for the whole run), which
+      // keeps both properties.  `prev` is held locked meanwhile (a merge that would remove
+      // it waits, then sees its longer run).  When the table cannot be rewritten (none
+      // given, `prev` busy, too long a run), the merge is counted, which retires the table.
+      bool patched = false;
+      u32 held = kInvalid;
+      if (st.layer == 0) {
+        const u32 prev = c.shortcut_prev(low);
+        if (prev == left) {
+          patched = c.shortcut_patch(right, prev, low);
+        } else if (prev < c.slab_count()) {
+          u32 pold;
+          if (c.try_lock(prev, pold)) {
+            held = prev;
+            patched = c.shortcut_patch(right, prev, low);
+          }
+        }
+      }
       c.fence_release();
+      if (held != kInvalid) c.unlock_same(held);
       c.unlock_bump(left);
       c.unlock_bump(st.node);
       c.unlock_bump(right);

@@ -61,7 +61,12 @@ struct cpu_tile {
   }
   u32 max_op_time() const { return P.max_op_time; }
   void count_merge() { P.merges.fetch_add(1); }
-  // the device policy's shortcut_patch on the model's table
+  // the device policy's shortcut_prev / shortcut_patch on the model's table
+  u32 slab_count() const { return P.num_slabs; }
+  u32 shortcut_prev(u32 low) const {
+    if (!P.sc_border) return kInvalid;
+    return __atomic_load_n(P.sc_border + (low >> P.sc_shift), __ATOMIC_RELAXED);
+  }
   bool shortcut_patch(u32 right, u32 left, u32 low) {
     if (!P.sc_border) return false;
     const u32 entries = 1u << (32u - P.sc_shift);

@@ -349,6 +349,57 @@ def test_merges_rewrite_shortcut_table(nthreads):
         assert b.m.check_shortcut() == 0, rnd
 
 
+@pytest.mark.parametrize("nthreads,ntiles", [(0, 1), (0, 64), (8, 64)],
+                         ids=["one_tile", "interleaved", "threads"])
+def test_shortcut_table_through_split_and_merge_churn(nthreads, ntiles):
+    """Rounds of inserts of fresh keys (splits make nodes that no entry names) and erases of
+    short contiguous ranges (runs of neighbouring nodes empty and merge, new nodes into old
+    ones and old into new): after every batch in which no merge had to retire the border-
+    level table, every entry is a live layer-0 border node at or left of its prefix's node.
+    One tile at a time, no merge ever finds the node it rewrites to busy, so the table is
+    never retired; with tiles running together a busy one retires it, and it is rebuilt."""
+    rng = np.random.default_rng(79)
+    mk, mv = 1, 1
+    live = set()
+    b = S.CpuModelBackend(nthreads=nthreads, ntiles=ntiles)
+    o = S.OracleBackend()
+    _eq = np.array_equal
+
+    def insert(k):
+        k = np.array(sorted(set(k.tolist()) - live), np.uint32)
+        if len(k) == 0:
+            return
+        keys, lens = k.reshape(-1, 1), np.ones(len(k), np.uint32)
+        vals = rng.integers(0, 1 << 32, size=(len(k), mv), dtype=np.uint64).astype(np.uint32)
+        assert _eq(b.insert(keys, lens, vals, None, mk, mv), o.insert(keys, lens, vals, None, mk, mv))
+        live.update(k.tolist())
+
+    insert(rng.integers(0, 1 << 32, size=3000, dtype=np.uint64).astype(np.uint32))
+    b.m.build_shortcut(10)
+    assert b.m.check_shortcut() == 0
+    checked = 0
+    for rnd in range(16):
+        insert(rng.integers(0, 1 << 32, size=800, dtype=np.uint64).astype(np.uint32))
+        assert b.m.check_shortcut() == 0, rnd  # splits leave every entry a valid start
+        retired = b.m.nodes(5)
+        lo = int(rng.integers(0, (1 << 32) - (1 << 26)))
+        k = np.array(sorted(x for x in live if lo <= x < lo + (1 << 26)), np.uint32)
+        if len(k):
+            keys, lens = k.reshape(-1, 1), np.ones(len(k), np.uint32)
+            assert _eq(b.erase(keys, lens, mk), o.erase(keys, lens, mk)), rnd
+            live.difference_update(k.tolist())
+        assert b.items() == o.items(), rnd
+        if b.m.nodes(5) == retired:
+            assert b.m.check_shortcut() == 0, rnd
+            checked += 1
+        else:
+            b.m.build_shortcut(10)  # retired: a fresh table, as the next rebuild gives
+    if ntiles == 1:
+        assert checked == 16
+    else:
+        assert checked >= 4
+
+
 def test_long_table_run_retires_shortcut_table():
     """When the entries naming the node a merge removes are a run longer than the merge
     rewrites (here the tree's last border node names every prefix above its keys), the
