@@ -223,7 +223,8 @@ int gmt_reserve_find_scratch(gmt_index* index, uint64_t max_requests);
  * merge it cannot follow retires until the next rebuild)
  * instead of the root: on (the default) or off.  An extension; results do not depend on it.  The table
  * is rebuilt when the insert count has doubled; one left from before later writes is
- * still correct. */
+ * still correct.  Built for the 16-lane tile width only (the instance that owns the tree);
+ * at 32 and 64 lanes descents start at the root. */
 int gmt_set_find_shortcut(gmt_index* index, int on);
 /* log2 of the entries of the largest shortcut table the index holds (0: none). */
 uint32_t gmt_find_shortcut_bits(gmt_index* index);

@@ -105,7 +105,8 @@ struct gpu_masstree {
         barrier_(other.barrier_),
         inserted_hint_(other.inserted_hint_),
         owner_(false),
-        fanout_flags_(other.fanout_flags_) {}
+        fanout_flags_(other.fanout_flags_),
+        shortcut_on_(false) {}
   template <typename, typename, uint32_t>
   friend struct gpu_masstree;
   // Attaches to the tree of an index with another tile width: the node layout does not
@@ -119,7 +120,8 @@ struct gpu_masstree {
         barrier_(other.barrier_),
         inserted_hint_(other.inserted_hint_),
         owner_(false),
-        fanout_flags_(other.fanout_flags_) {}
+        fanout_flags_(other.fanout_flags_),
+        shortcut_on_(false) {}
   gpu_masstree& operator=(const gpu_masstree&) = delete;
   ~gpu_masstree() {
     if (scan_stack_) (void)hipFree(scan_stack_);  // per instance, never shared
@@ -390,8 +392,13 @@ struct gpu_masstree {
   // start hints of write batches) start at a layer-0 shortcut table (refresh_shortcut)
   // instead of the root: on (the default) or off (the table is dropped).  An extension;
   // results do not depend on it.
+  // Only the instance that owns the tree builds shortcut tables: merges rewrite the
+  // border-level table of the launch that makes them, so a second table (of an instance
+  // attached with another tile width) would miss the merges of the first one's writes.
+  // The attached instances' writes give no table and count their merges, which retires
+  // the owner's border-level table until its next rebuild.
   void set_find_shortcut(bool on) {
-    shortcut_on_ = on;
+    shortcut_on_ = on && owner_;
     if (!on && shortcut_) {
       IGPU_HIP_TRY(hipDeviceSynchronize());
       IGPU_HIP_TRY(hipFree(shortcut_));
