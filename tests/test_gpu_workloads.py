@@ -537,3 +537,32 @@ def test_find_shortcut_table(find_tile):
         _eq(a, b)
     ok, *_, msg = g.validate()
     assert ok, msg
+
+
+def test_shortcut_tables_belong_to_the_owning_instance():
+    """Only the 16-lane instance (the one that owns the tree) builds shortcut tables: inserts
+    and finds at 32 lanes build none, finds at 16 lanes build them, erases at 32 lanes (their
+    launches have no table, so their layer-0 merges are counted and retire the border-level
+    table) leave finds at 16 lanes equal to the oracle's."""
+    rng = np.random.default_rng(29)
+    n, mk, mv = 400_000, 2, 1
+    keys, lens = W.unique_random_keys(rng, n, 1, mk)
+    keys = np.where(np.arange(mk)[None, :] < lens[:, None], keys, 0).astype(np.uint32)
+    vals = rng.integers(0, 1 << 32, size=(n, mv), dtype=np.uint64).astype(np.uint32)
+    g, o = S.GpuBackend(32, pool_bytes=1 << 30), S.OracleBackend()
+    _eq(g.insert(keys, lens, vals, None, mk, mv), o.insert(keys, lens, vals, None, mk, mv))
+    for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        _eq(a, b)
+    assert g.t.find_shortcut_bits() == 0
+    g.t.tile_size = 16
+    for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        _eq(a, b)
+    assert g.t.find_shortcut_bits() > 0
+    g.t.tile_size = 32
+    low = np.nonzero(keys[:, 0] < (1 << 28))[0]
+    _eq(g.erase(keys[low], lens[low], mk), o.erase(keys[low], lens[low], mk))
+    g.t.tile_size = 16
+    for a, b in zip(g.find(keys, lens, mk, mv), o.find(keys, lens, mk, mv)):
+        _eq(a, b)
+    ok, *_, msg = g.validate()
+    assert ok, msg
