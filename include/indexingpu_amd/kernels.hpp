@@ -131,6 +131,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 batch_kernel(tree_view tv, batch_args a) {
   wave_tile<T, Validate> c(tv.pool, tv.retire, tv.err, tv.root);
   c.sc_border = tv.shortcut_border;
+  c.sc_epoch = tv.shortcut_epoch;
   c.sc_shift = tv.shortcut_shift;
   const u32 tiles_per_block = kBlockThreads / T;
   const u64 ntiles = (u64)gridDim.x * tiles_per_block;
@@ -268,6 +269,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 multi_batch_kernel(tree_view tv, multi_args m) {
   wave_tile<T, true> c(tv.pool, tv.retire, tv.err, tv.root);
   c.sc_border = tv.shortcut_border;
+  c.sc_epoch = tv.shortcut_epoch;
   c.sc_shift = tv.shortcut_shift;
   const batch_args& a = m.a;
   const u32 tiles_per_block = kBlockThreads / T;

@@ -111,13 +111,17 @@ struct wave_tile {
   // `low`, to `target` (tree_ops.hpp, ph_merge_lock).  shortcut_prev: the entry of the
   // prefix holding `low` (kInvalid: no table).  shortcut_patch: false when there is no
   // table or the run is longer than kShortcutPatchMax.
+  // A table already retired (a merge counted since its build) is not rewritten: its
+  // entries may name nodes freed since, so shortcut_prev gives kInvalid for it too.
   u32* sc_border = nullptr;
+  const u32* sc_epoch = nullptr;  // the merge count the table was built at
   u32 sc_shift = 0;
   __device__ __forceinline__ u32 slab_count() const { return pool.num_slabs; }
   __device__ __forceinline__ u32 shortcut_prev(u32 low) const {
     if (!sc_border) return kInvalid;
-    u32 v = 0;
-    if (lane == 0) v = dev::coherent_load(sc_border + (low >> sc_shift));
+    u32 v = kInvalid;
+    if (lane == 0 && dev::coherent_load(err + kMergeEpochWord) == dev::coherent_load(sc_epoch))
+      v = dev::coherent_load(sc_border + (low >> sc_shift));
     return bcast(v, 0);
   }
   __device__ __forceinline__ bool shortcut_patch(u32 right, u32 left, u32 low) {

@@ -250,6 +250,7 @@ void cm_build_shortcut(void* h, uint32_t bits) {
   for (u32 i = 0; i < entries; i++) M.shortcut[i] = border_of(M, i << shift);
   M.pool.sc_border = M.shortcut.data();
   M.pool.sc_shift = shift;
+  M.pool.sc_built = M.pool.merges.load();
 }
 // Entries of the table that are no valid start for their prefix: not a live layer-0
 // border node (on the chain of sibling links from the leftmost one), or right of the

@@ -37,6 +37,7 @@ struct shared_pool {
   std::atomic<u32> merges{0};     // layer-0 border nodes merged away without a table rewrite
   u32* sc_border = nullptr;       // border-level shortcut table (null: none) and its shift
   u32 sc_shift = 0;
+  u32 sc_built = 0;               // `merges` when the table was built
   u32 trip[kTripWords] = {};     // the guard's record of the operation it abandoned
   explicit shared_pool(u32 n) : slots((size_t)n * kSlots, 0), num_slabs(n) {}
   u64* slab(u32 idx) { return &slots[(size_t)idx * kSlots]; }
@@ -64,7 +65,7 @@ struct cpu_tile {
   // the device policy's shortcut_prev / shortcut_patch on the model's table
   u32 slab_count() const { return P.num_slabs; }
   u32 shortcut_prev(u32 low) const {
-    if (!P.sc_border) return kInvalid;
+    if (!P.sc_border || P.merges.load() != P.sc_built) return kInvalid;
     return __atomic_load_n(P.sc_border + (low >> P.sc_shift), __ATOMIC_RELAXED);
   }
   bool shortcut_patch(u32 right, u32 left, u32 low) {
