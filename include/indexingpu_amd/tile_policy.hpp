@@ -124,7 +124,7 @@ struct wave_tile {
       v = dev::coherent_load(sc_border + (low >> sc_shift));
     return bcast(v, 0);
   }
-  __device__ __forceinline__ bool shortcut_patch(u32 right, u32 left, u32 low) {
+  __device__ __forceinline__ bool shortcut_patch(u32 right, u32 target, u32 low) {
     if (!sc_border) return false;
     u32 ok = 0;
     if (lane == 0) {
@@ -132,7 +132,7 @@ struct wave_tile {
       u32 i = (low >> sc_shift) + 1u;  // entries at or below low name `left` or nodes left of it
       u32 n = 0;
       while (i < entries && n < kShortcutPatchMax && dev::coherent_load(sc_border + i) == right) {
-        dev::coherent_store(sc_border + i, left);
+        dev::coherent_store(sc_border + i, target);
         i++;
         n++;
       }

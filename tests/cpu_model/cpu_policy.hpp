@@ -68,14 +68,14 @@ struct cpu_tile {
     if (!P.sc_border || P.merges.load() != P.sc_built) return kInvalid;
     return __atomic_load_n(P.sc_border + (low >> P.sc_shift), __ATOMIC_RELAXED);
   }
-  bool shortcut_patch(u32 right, u32 left, u32 low) {
+  bool shortcut_patch(u32 right, u32 target, u32 low) {
     if (!P.sc_border) return false;
     const u32 entries = 1u << (32u - P.sc_shift);
     u32 i = (low >> P.sc_shift) + 1u;
     u32 n = 0;
     while (i < entries && n < kShortcutPatchMax &&
            __atomic_load_n(P.sc_border + i, __ATOMIC_RELAXED) == right) {
-      __atomic_store_n(P.sc_border + i, left, __ATOMIC_RELAXED);
+      __atomic_store_n(P.sc_border + i, target, __ATOMIC_RELAXED);
       i++;
       n++;
     }
